@@ -1,0 +1,258 @@
+"""Periodic linked-cell k-nearest search on MI355X (gfx950).
+
+Python surface of the drop-in library, mirroring the reference module
+``linkcell`` (linkcell/__init__.py:1-11, python/src/lib.rs:318-417):
+
+* ``knearest(xyz, cell, k, mask=None, cell_hint=None) -> (indices, dist2)``
+* ``gpu_available() -> bool``
+* ``__version__``
+
+plus ``Workspace``, the Python face of ``linkcell::gpu::Workspace``.
+
+Host (numpy / any CPU ``__dlpack__``) inputs go through the ``lc_knearest``
+C API, device (torch on ROCm, DLPack device type 10, or CUDA type 2)
+inputs through ``lc_gpu_knearest_many`` on device pointers. Both run the
+HIP kernels; results are bit-identical to the reference CPU walk.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+__all__ = ["__version__", "gpu_available", "knearest", "Workspace", "Error"]
+
+# DLPack device types (dlpack.h): CPU, CUDA, CUDAHost, ROCM, ROCMHost.
+_DL_CPU, _DL_CUDA, _DL_CUDA_HOST, _DL_ROCM, _DL_ROCM_HOST = 1, 2, 3, 10, 11
+
+
+class Error(RuntimeError):
+    """A failed search; the message is the C waist's error string."""
+
+
+def _lib_handle():
+    return _lib.load()
+
+
+__version__ = _lib_handle().lc_version().decode()
+
+
+def gpu_available() -> bool:
+    return bool(_lib_handle().lc_gpu_available())
+
+
+def _cell12(cell) -> list:
+    """Cell shapes of python/src/lib.rs:235-308: (3,) ortho lengths, (3, 3)
+    lattice rows, (4, 3) rows plus origin."""
+    c = np.asarray(cell, dtype=np.float64)
+    if c.shape == (3,):
+        return [c[0], 0.0, 0.0, 0.0, c[1], 0.0, 0.0, 0.0, c[2], 0.0, 0.0, 0.0]
+    if c.shape == (3, 3):
+        return list(c.reshape(9)) + [0.0, 0.0, 0.0]
+    if c.shape == (4, 3):
+        return list(c.reshape(12))
+    raise ValueError("cell must be float64 shape (3,) ortho, (3, 3) lattice rows, "
+                     "or (4, 3) rows plus origin")
+
+
+def _xyz_shape(shape) -> tuple:
+    """(n, n_frames) for shapes (n, 3), (F, n, 3), (3n,)
+    (python/src/lib.rs:213-233)."""
+    shape = tuple(int(s) for s in shape)
+    if len(shape) == 2 and shape[1] == 3 and shape[0] > 0:
+        return shape[0], 1
+    if len(shape) == 3 and shape[2] == 3 and shape[0] > 0 and shape[1] > 0:
+        return shape[1], shape[0]
+    if len(shape) == 1 and shape[0] > 0 and shape[0] % 3 == 0:
+        return shape[0] // 3, 1
+    raise ValueError("xyz must be float64 shape (n, 3), (n_frames, n, 3), or (n*3,)")
+
+
+def _device_type(obj) -> int:
+    dev = getattr(obj, "__dlpack_device__", None)
+    if dev is None:
+        return _DL_CPU
+    return int(dev()[0])
+
+
+def _is_torch(obj) -> bool:
+    return type(obj).__module__.startswith("torch")
+
+
+class Workspace:
+    """One HIP stream plus grow-only device buffers
+    (include/linkcell_gpu.hpp:29-78)."""
+
+    def __init__(self):
+        lib = _lib_handle()
+        self._lib = lib
+        self._ws = lib.lc_gpu_workspace_new()
+        if not self._ws:
+            msg = lib.lc_gpu_last_error()
+            raise Error(msg.decode() if msg else "workspace creation failed")
+
+    def close(self):
+        if getattr(self, "_ws", None):
+            self._lib.lc_gpu_workspace_free(self._ws)
+            self._ws = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self._lib.lc_gpu_last_error()
+            raise Error(msg.decode() if msg else "device search failed")
+
+    @property
+    def queue(self) -> int:
+        return int(self._lib.lc_gpu_queue(self._ws) or 0)
+
+    def wait(self):
+        self._lib.lc_gpu_wait(self._ws)
+
+    def set_mode(self, mode: int):
+        self._check(self._lib.lc_gpu_set_mode(self._ws, int(mode)))
+
+    def set_timing(self, on: bool):
+        self._check(self._lib.lc_gpu_set_timing(self._ws, 1 if on else 0))
+
+    def last_timing(self) -> dict:
+        ms = (C.c_double * 7)()
+        deferred = C.c_int(0)
+        self._check(self._lib.lc_gpu_last_timing(self._ws, ms, C.byref(deferred)))
+        keys = ("fold", "scan", "scatter", "sort", "walk", "exact", "total")
+        out = {k: float(v) for k, v in zip(keys, ms)}
+        out["deferred"] = int(deferred.value)
+        return out
+
+    def knearest_ptr(self, xyz_ptr: int, n: int, n_frames: int, cell12, k: int,
+                     out_nn_ptr: int, out_d2_ptr: int = 0, mask_ptr: int = 0,
+                     cell_hint: float = 0.0, out_reach_ptr: int = 0,
+                     wait: bool = True):
+        """Raw device-pointer search (``lc_gpu_knearest_ex``)."""
+        cell = _lib.lc_cell.from12(cell12)
+        rc = self._lib.lc_gpu_knearest_ex(
+            self._ws, C.c_void_p(xyz_ptr), n, n_frames, C.byref(cell), k,
+            C.c_void_p(mask_ptr or None), float(cell_hint),
+            C.c_void_p(out_nn_ptr), C.c_void_p(out_d2_ptr or None),
+            C.c_void_p(out_reach_ptr or None), 1 if wait else 0)
+        self._check(rc)
+
+    def knearest_dcell_ptr(self, xyz_ptr: int, n: int, n_frames: int,
+                           cell_ptr: int, cell_n: int, k: int, out_nn_ptr: int,
+                           out_d2_ptr: int = 0, mask_ptr: int = 0,
+                           cell_hint: float = 0.0, wait: bool = True):
+        rc = self._lib.lc_gpu_knearest_many_dcell(
+            self._ws, C.c_void_p(xyz_ptr), n, n_frames, C.c_void_p(cell_ptr),
+            int(cell_n), k, C.c_void_p(mask_ptr or None), float(cell_hint),
+            C.c_void_p(out_nn_ptr), C.c_void_p(out_d2_ptr or None),
+            1 if wait else 0)
+        self._check(rc)
+
+
+_default_ws = None
+
+
+def _workspace() -> Workspace:
+    global _default_ws
+    if _default_ws is None:
+        _default_ws = Workspace()
+    return _default_ws
+
+
+def _host_mask(mask, n):
+    if mask is None:
+        return None
+    m = np.asarray(mask)
+    if m.size != n:
+        raise ValueError("mask length must be n")
+    if m.dtype == np.int32:
+        return np.ascontiguousarray(m.reshape(n))
+    if m.dtype == np.float64:
+        return np.ascontiguousarray((m.reshape(n) != 0.0).astype(np.int32))
+    raise ValueError("mask must be int32 or float64")
+
+
+def knearest(xyz, cell, k, mask=None, cell_hint=None):
+    """k nearest periodic neighbours of every point.
+
+    ``xyz`` is float64 ``(n, 3)``, ``(n_frames, n, 3)`` or ``(n*3,)``;
+    ``cell`` is ``(3,)``, ``(3, 3)`` or ``(4, 3)`` float64 on any device;
+    ``mask`` (length n, int32 or float64, nonzero keeps) drops points as
+    source and candidate. Returns ``(indices, dist2)``: int32 with -1 and
+    float64 with NaN in unused slots, shape ``(n, k)`` or
+    ``(n_frames, n, k)``. Device inputs return device tensors.
+    """
+    k = int(k)
+    if k <= 0:
+        raise ValueError("k must be at least 1")
+    hint = float(cell_hint) if cell_hint is not None else 0.0
+    if _device_type(xyz) in (_DL_ROCM, _DL_CUDA):
+        return _knearest_device(xyz, cell, k, mask, hint)
+    if _is_torch(cell) and _device_type(cell) in (_DL_ROCM, _DL_CUDA):
+        cell = cell.detach().cpu().numpy()
+    x = np.asarray(xyz)
+    if x.dtype != np.float64:
+        raise ValueError("xyz must be float64 shape (n, 3), (n_frames, n, 3), or (n*3,)")
+    n, frames = _xyz_shape(x.shape)
+    x = np.ascontiguousarray(x)
+    c12 = _cell12(cell)
+    m = _host_mask(mask, n)
+    nn = np.full(frames * n * k, -1, dtype=np.int32)
+    d2 = np.full(frames * n * k, np.nan, dtype=np.float64)
+    lib = _lib_handle()
+    cobj = _lib.lc_cell.from12(c12)
+    rc = lib.lc_knearest_many(
+        x.ctypes.data, n, frames, C.byref(cobj), k,
+        m.ctypes.data if m is not None else None, hint, nn.ctypes.data,
+        d2.ctypes.data)
+    if rc != 0:
+        msg = lib.lc_last_error()
+        raise Error(msg.decode() if msg else "knearest failed")
+    shape = (n, k) if frames == 1 else (frames, n, k)
+    return nn.reshape(shape), d2.reshape(shape)
+
+
+def _knearest_device(xyz, cell, k, mask, hint):
+    import torch
+
+    if not _is_torch(xyz):
+        xyz = torch.from_dlpack(xyz)
+    if xyz.dtype != torch.float64:
+        raise ValueError("xyz must be float64 shape (n, 3), (n_frames, n, 3), or (n*3,)")
+    n, frames = _xyz_shape(xyz.shape)
+    xyz = xyz.contiguous()
+    dev = xyz.device
+    shape = (n, k) if frames == 1 else (frames, n, k)
+    nn = torch.empty(shape, dtype=torch.int32, device=dev)
+    d2 = torch.empty(shape, dtype=torch.float64, device=dev)
+    m_dev = None
+    if mask is not None:
+        if _is_torch(mask) and mask.is_cuda:
+            m_dev = mask.to(torch.float64) if mask.dtype == torch.float64 else mask
+            m_dev = (m_dev != 0).to(torch.int32).contiguous() if m_dev.dtype != torch.int32 else m_dev.contiguous()
+        else:
+            m_dev = torch.from_numpy(_host_mask(mask, n)).to(dev)
+        if m_dev.numel() != n:
+            raise ValueError("mask length must be n")
+    ws = _workspace()
+    torch.cuda.current_stream(dev).synchronize()
+    if _is_torch(cell) and cell.is_cuda:
+        c = cell.to(torch.float64).contiguous()
+        if tuple(c.shape) not in ((3,), (3, 3), (4, 3)):
+            raise ValueError("cell must be float64 shape (3,) ortho, (3, 3) lattice rows, "
+                             "or (4, 3) rows plus origin")
+        ws.knearest_dcell_ptr(xyz.data_ptr(), n, frames, c.data_ptr(), c.numel(), k,
+                              nn.data_ptr(), d2.data_ptr(),
+                              m_dev.data_ptr() if m_dev is not None else 0, hint)
+    else:
+        ws.knearest_ptr(xyz.data_ptr(), n, frames, _cell12(cell), k, nn.data_ptr(),
+                        d2.data_ptr(), m_dev.data_ptr() if m_dev is not None else 0,
+                        hint)
+    return nn, d2

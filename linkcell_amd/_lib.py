@@ -1,0 +1,94 @@
+"""ctypes binding of the in-tree HIP library (``liblinkcell_amd.so``).
+
+Loading fails loudly when the library has not been built: there is no
+CPU fallback behind any entry point of this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "liblinkcell_amd.so"
+
+# Every symbol the public headers under include/ declare.
+EXPORTS = (
+    # include/linkcell.h
+    "lc_knearest", "lc_knearest_d2", "lc_knearest_many", "lc_last_error",
+    "lc_version",
+    # include/linkcell_gpu.h
+    "lc_gpu_available", "lc_gpu_last_error", "lc_gpu_workspace_new",
+    "lc_gpu_workspace_free", "lc_gpu_knearest", "lc_gpu_knearest_many",
+    "lc_gpu_knearest_many_dcell", "lc_gpu_queue", "lc_gpu_wait",
+    "lc_gpu_alloc", "lc_gpu_free", "lc_gpu_fill_i32", "lc_gpu_memcpy",
+    # include/linkcell_amd.h
+    "lc_gpu_knearest_ex", "lc_gpu_set_mode", "lc_gpu_set_timing",
+    "lc_gpu_last_timing",
+)
+
+
+class lc_cell(C.Structure):
+    """Mirror of ``struct lc_cell`` (rows a, b, c then origin)."""
+
+    _fields_ = [(name, C.c_double) for name in (
+        "ax", "ay", "az", "bx", "by", "bz", "cx", "cy", "cz", "ox", "oy", "oz")]
+
+    @classmethod
+    def from12(cls, v):
+        v = [float(x) for x in v]
+        if len(v) != 12:
+            raise ValueError("lc_cell needs 12 values")
+        return cls(*v)
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+_sz = C.c_size_t
+
+_PROTOS = {
+    "lc_knearest": (C.c_int, [_vp, _sz, C.POINTER(lc_cell), _sz, _vp, C.c_double, _vp]),
+    "lc_knearest_d2": (C.c_int, [_vp, _sz, C.POINTER(lc_cell), _sz, _vp, C.c_double, _vp, _vp]),
+    "lc_knearest_many": (C.c_int, [_vp, _sz, _sz, C.POINTER(lc_cell), _sz, _vp, C.c_double, _vp, _vp]),
+    "lc_last_error": (C.c_char_p, []),
+    "lc_version": (C.c_char_p, []),
+    "lc_gpu_available": (C.c_int, []),
+    "lc_gpu_last_error": (C.c_char_p, []),
+    "lc_gpu_workspace_new": (_vp, []),
+    "lc_gpu_workspace_free": (None, [_vp]),
+    "lc_gpu_knearest": (C.c_int, [_vp, _vp, _sz, C.POINTER(lc_cell), _sz, _vp, C.c_double, _vp, _vp]),
+    "lc_gpu_knearest_many": (C.c_int, [_vp, _vp, _sz, _sz, C.POINTER(lc_cell), _sz, _vp,
+                                       C.c_double, _vp, _vp, C.c_int]),
+    "lc_gpu_knearest_many_dcell": (C.c_int, [_vp, _vp, _sz, _sz, _vp, C.c_int, _sz, _vp,
+                                             C.c_double, _vp, _vp, C.c_int]),
+    "lc_gpu_queue": (_vp, [_vp]),
+    "lc_gpu_wait": (None, [_vp]),
+    "lc_gpu_alloc": (C.c_int, [C.POINTER(_vp), _sz]),
+    "lc_gpu_free": (None, [_vp]),
+    "lc_gpu_fill_i32": (C.c_int, [_vp, C.c_int, _sz]),
+    "lc_gpu_memcpy": (C.c_int, [_vp, _vp, _sz, C.c_int]),
+    "lc_gpu_knearest_ex": (C.c_int, [_vp, _vp, _sz, _sz, C.POINTER(lc_cell), _sz, _vp,
+                                     C.c_double, _vp, _vp, _vp, C.c_int]),
+    "lc_gpu_set_mode": (C.c_int, [_vp, C.c_int]),
+    "lc_gpu_set_timing": (C.c_int, [_vp, C.c_int]),
+    "lc_gpu_last_timing": (C.c_int, [_vp, _dp, _ip]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the library once; raise if it is missing (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m linkcell_amd.build` "
+            "(the HIP library is the only implementation)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

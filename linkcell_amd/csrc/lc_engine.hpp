@@ -1,0 +1,112 @@
+// Host-side engine behind linkcell::gpu::Workspace and the lc_gpu_* waist.
+//
+// One engine owns one HIP stream and grow-only device buffers (the
+// reference Workspace::Impl, src/gpu/knearest.cpp:571-674). A call runs a
+// batch of frames; each frame has its own points, cell, mesh and k, so
+// the shared-cell entry points (src/gpu/knearest.cpp:705-816) and the
+// batched uneven-frame path use the same kernels.
+#pragma once
+
+#include "lc_cell.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <string>
+#include <vector>
+
+namespace lcamd {
+
+// Device-side frame descriptor.
+struct FrameDesc {
+  const double* xyz;  // n x 3, AoS, device
+  const int* mask;    // n ints or null, device
+  int* out_nn;        // n x k, device
+  double* out_d2;     // n x k or null, device
+  int* out_reach;     // n or null (final shell per source; diagnostics)
+  double h[9];
+  double hinv[9];
+  double origin[3];
+  double widths[3];
+  double cell_min;
+  long long pt_base;    // first global point id of the frame
+  long long cell_base;  // first global cell id of the frame
+  int n;
+  int k;
+  int nx, ny, nz;
+  int max_reach;
+  int r_safe;  // largest reach free of stencil wrap (see safe_reach)
+  int ortho;
+  int kc;  // register-heap bucket for the fast walk, 0 = exact walk only
+};
+
+// Host description of one frame of a call.
+struct FrameSpec {
+  const double* xyz = nullptr;
+  const int* mask = nullptr;
+  int* out_nn = nullptr;
+  double* out_d2 = nullptr;
+  int* out_reach = nullptr;
+  int n = 0;
+  int k = 0;
+  CellGeom geom{};
+  MeshPlan plan{};
+};
+
+// Per-call kernel timings in milliseconds (hipEvent pairs on the engine
+// stream), valid after a call made with timing on and a wait.
+struct Timings {
+  float fold = 0, scan = 0, scatter = 0, sort = 0, walk = 0, exact = 0,
+        total = 0;
+  int deferred = 0;  // sources finished by the exact-order walk
+};
+
+enum class WalkMode : int {
+  kAuto = 0,       // canonical fast walk, exact-order walk for the rest
+  kExactOnly = 1,  // every source through the exact-order walk
+};
+
+class Engine {
+ public:
+  Engine();
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  hipStream_t stream();
+  void sync();
+  // Runs a batch. Throws std::runtime_error on HIP failures.
+  void run(const std::vector<FrameSpec>& frames, bool wait);
+
+  void set_timing(bool on) { timing_ = on; }
+  const Timings& last_timings();
+  void set_mode(WalkMode m) { mode_ = m; }
+
+ private:
+  struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+  };
+  void grow(Buf& b, size_t bytes);
+  void free_all();
+
+  hipStream_t stream_ = nullptr;
+  int device_ = -1;
+  Buf frames_, fpt_, fcb_, count_, status_, counters_, off_, item_off_,
+      items_, cellslot_, tmp_idx_, sx_, sy_, sz_, sidx_, scell_, defer_,
+      heap_d2_, heap_idx_, flags_;
+  void* staging_ = nullptr;
+  size_t staging_bytes_ = 0;
+  hipEvent_t staging_ev_ = nullptr;
+  bool staging_pending_ = false;
+  bool timing_ = false;
+  bool timings_pending_ = false;
+  WalkMode mode_ = WalkMode::kAuto;
+  hipEvent_t ev_[8] = {};
+  Timings timings_{};
+};
+
+// Heap-capacity bucket of the register-resident fast walk for k.
+int fast_bucket(int k);
+
+}  // namespace lcamd
