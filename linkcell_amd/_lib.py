@@ -85,6 +85,14 @@ def load():
         raise ImportError(
             f"{LIB_PATH} is not built; run `python -m linkcell_amd.build` "
             "(the HIP library is the only implementation)")
+    # PyTorch-ROCm ships its own libamdhip64.so.7 / libhsa-runtime64.so.1.
+    # Loading it first makes the dynamic linker bind this library to the
+    # same runtime (same sonames); loading /opt/rocm's copy first would put
+    # two HIP runtimes in the process and torch's device init then fails.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in _PROTOS.items():
         fn = getattr(lib, name)
