@@ -129,12 +129,16 @@ def main():
     ap.add_argument("--config", default="c3")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", type=int, default=0)
+    ap.add_argument("--dense-min", type=int, default=None,
+                    help="home-cell occupancy at which the wave-per-cell walk takes over")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding is a "
               "design only (see DESIGN.md), so --gpus must be 1", file=sys.stderr)
         raise SystemExit(2)
 
+    if args.dense_min is not None:
+        os.environ["LINKCELL_DENSE_MIN"] = str(args.dense_min)
     import torch
     import linkcell_amd as lc
 

@@ -1,22 +1,33 @@
 // Linked-cell k-nearest pipeline on gfx950 (MI355X).
 //
+// Preparation (all single-pass streaming kernels):
 //   fold_count  fold every point into its primary cell, bin it, count
-//               occupants with one atomic per point (the slot it returns
-//               is the point's arrival rank in the cell)
-//   scan        single-pass decoupled look-back exclusive scan over cell
-//               counts; one packed 64-bit word per tile carries the
-//               occupant prefix and the 64-source chunk prefix together
-//   scatter     arrival-order scatter of point ids into cell segments
-//   cell_sort   per cell chunk: rank by original index, re-fold, write
-//               SoA FP64 coordinates, ids and home cells in cell order
-//               with ascending ids inside each cell
-//   walk_fast   one lane per source; Chebyshev shells with a canonical
-//               (d2, index) register heap and the exact stop test
-//   walk_exact  exact-order emulation of the reference KHeap for the
-//               sources the fast walk defers (boundary ties, stencil
-//               wrap, k above the register buckets, non-finite input)
+//               occupants with one atomic per point (the slot it returns is
+//               the point's arrival rank in the cell); a 6-bit Morton key of
+//               the point's 4x4x4 sub-cell rides along
+//   scan x2     single-pass decoupled look-back exclusive scans over cell
+//               counts; one packed 64-bit status word per 2048-cell tile
+//               carries the occupant prefix and a second prefix (16-point
+//               tiles, then 64-source chunks)
+//   scatter     arrival-order scatter of (key, id) into cell segments
+//   cell_sort   one wave per 64-member chunk: rank by (Morton key, id) for
+//               the walk layout and by id for the exact-order replay,
+//               re-fold, write SoA FP64 coordinates
+//   boxes       exact bounding boxes of every 16-point tile and every cell
+// Walks:
+//   walk_dense  one wave per 64 sources of one dense home cell; candidates
+//               arrive through scalar loads (uniform across the wave), tiles
+//               whose exact lower-bound distance exceeds every lane's k-th
+//               best are skipped, accepted candidates are buffered in LDS and
+//               merged into per-lane register heaps in batches
+//   walk_sparse one lane per source of a sparse home cell, same heap, the
+//               same tile bounds per lane
+//   walk_exact  exact-order replay of the reference walk and KHeap for the
+//               sources the fast walks defer (distance ties at or inside the
+//               k-th, stencil wrap, k above the register buckets, non-finite
+//               input)
 //
-// Reference: src/knearest.rs:351-486 (CPU walk, the parity target) and
+// Reference: src/knearest.rs:351-486 (the CPU walk, the parity target) and
 // src/gpu/knearest.cpp:192-441 (the CUDA kernels this file replaces).
 #include "lc_engine.hpp"
 
@@ -24,6 +35,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -45,6 +57,23 @@ constexpr int kFrameNonFinite = 1;
 constexpr long long kInfBits = 0x7ff0000000000000LL;
 constexpr long long kUnusedBits = LLONG_MIN;
 constexpr unsigned long long kNanBits = 0x7ff8000000000000ull;
+constexpr int kTile = 16;      // points per bounding-box tile
+constexpr int kTileShift = 4;
+constexpr int kChunkShift = 6;  // sources per dense work item (one wave)
+constexpr int kBuf = 16;        // per-lane LDS candidate buffer entries
+constexpr int kDenseWaves = 4;  // waves per dense-walk block
+constexpr int kPad = 16;        // slack elements after SoA arrays (group reads)
+
+// r = 1 stencil in near-first order: home, 6 faces, 12 edges, 8 corners.
+// The canonical heap makes the result independent of the visit order
+// inside a shell; the order only makes the k-th distance tight early.
+__constant__ signed char kNear27[27][3] = {
+    {0, 0, 0},   {-1, 0, 0},  {1, 0, 0},   {0, -1, 0},  {0, 1, 0},
+    {0, 0, -1},  {0, 0, 1},   {-1, -1, 0}, {-1, 1, 0},  {1, -1, 0},
+    {1, 1, 0},   {-1, 0, -1}, {-1, 0, 1},  {1, 0, -1},  {1, 0, 1},
+    {0, -1, -1}, {0, -1, 1},  {0, 1, -1},  {0, 1, 1},   {-1, -1, -1},
+    {-1, -1, 1}, {-1, 1, -1}, {-1, 1, 1},  {1, -1, -1}, {1, -1, 1},
+    {1, 1, -1},  {1, 1, 1}};
 
 void check(hipError_t e, const char* what) {
   if (e != hipSuccess) {
@@ -81,6 +110,27 @@ __device__ __forceinline__ double nan_value() {
   return __longlong_as_double(static_cast<long long>(kNanBits));
 }
 
+__device__ __forceinline__ double as_f64(long long b) {
+  return __longlong_as_double(b);
+}
+
+__device__ __forceinline__ int uni(int v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// 2 bits of sub-cell position per axis, interleaved (z, y, x).
+__device__ __forceinline__ int sub_bits(double s, int n, int bin) {
+  const double u = s * static_cast<double>(n) - static_cast<double>(bin);
+  int v = static_cast<int>(u * 4.0);
+  v = v < 0 ? 0 : (v > 3 ? 3 : v);
+  return v;
+}
+
+__device__ __forceinline__ int morton6(int x, int y, int z) {
+  return (x & 1) | ((y & 1) << 1) | ((z & 1) << 2) | ((x & 2) << 2) |
+         ((y & 2) << 3) | ((z & 2) << 4);
+}
+
 // ---------------------------------------------------------------- fill
 
 __global__ void k_zero_u32(unsigned* p, long long n) {
@@ -97,7 +147,7 @@ __global__ void k_zero_u32(unsigned* p, long long n) {
 __global__ __launch_bounds__(256) void k_fold_count(
     const FrameDesc* __restrict__ fr, const long long* __restrict__ fpt,
     int F, long long ntot, int* __restrict__ count,
-    int2* __restrict__ cellslot, int* __restrict__ fflags) {
+    int4* __restrict__ cellslot, int* __restrict__ fflags) {
   const long long p = blockIdx.x * static_cast<long long>(blockDim.x) +
                       threadIdx.x;
   if (p >= ntot) return;
@@ -105,7 +155,7 @@ __global__ __launch_bounds__(256) void k_fold_count(
   const FrameDesc& d = fr[f];
   const int i = static_cast<int>(p - d.pt_base);
   if (d.mask != nullptr && d.mask[i] == 0) {
-    cellslot[p] = make_int2(-1, 0);
+    cellslot[p] = make_int4(-1, 0, 0, 0);
     const size_t row = static_cast<size_t>(i) * d.k;
     for (int t = 0; t < d.k; ++t) {
       d.out_nn[row + t] = -1;
@@ -124,17 +174,20 @@ __global__ __launch_bounds__(256) void k_fold_count(
   const int bx = bin_axis(s0, d.nx);
   const int by = bin_axis(s1, d.ny);
   const int bz = bin_axis(s2, d.nz);
+  const int key = morton6(sub_bits(s0, d.nx, bx), sub_bits(s1, d.ny, by),
+                          sub_bits(s2, d.nz, bz));
   const long long gcell =
       d.cell_base + (static_cast<long long>(bz) * d.ny + by) * d.nx + bx;
   const int slot = atomicAdd(&count[gcell], 1);
-  cellslot[p] = make_int2(static_cast<int>(gcell), slot);
+  cellslot[p] = make_int4(static_cast<int>(gcell), slot, key, 0);
 }
 
 // ---------------------------------------------------------------- scan
 
-__device__ __forceinline__ unsigned long long pack_cell(int cnt) {
-  const unsigned long long ch = static_cast<unsigned long long>((cnt + 63) >> 6);
-  return static_cast<unsigned long long>(cnt) | (ch << 31);
+__device__ __forceinline__ unsigned long long pack_cell(int cnt, int sh) {
+  const unsigned long long second =
+      static_cast<unsigned long long>((cnt + (1 << sh) - 1) >> sh);
+  return static_cast<unsigned long long>(cnt) | (second << 31);
 }
 
 __device__ __forceinline__ unsigned long long wave_sum_u64(
@@ -147,11 +200,12 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(
 // from an atomic counter, so a tile only ever waits on tiles that are
 // already resident. Each tile publishes its aggregate at once and its
 // inclusive prefix after the look-back; the whole first wave inspects 64
-// predecessors per step with one ballot.
+// predecessors per step with one ballot. The scanned value packs the
+// occupant count (bits 0-30) with ceil(count / 2^sh) (bits 31-61).
 __global__ __launch_bounds__(kScanThreads) void k_scan(
-    const int* __restrict__ count, long long ncells, int* __restrict__ off,
-    int* __restrict__ item_off, unsigned long long* status,
-    int* tile_counter) {
+    const int* __restrict__ count, long long ncells, int sh,
+    int* __restrict__ off, int* __restrict__ second_off,
+    unsigned long long* status, int* tile_counter) {
   __shared__ int s_tile;
   __shared__ unsigned long long s_wave[kScanThreads / kWave];
   __shared__ unsigned long long s_prefix;
@@ -169,7 +223,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(
     const long long idx = base + t;
     const int c = idx < ncells ? count[idx] : 0;
     ex[t] = sum;
-    sum += pack_cell(c);
+    sum += pack_cell(c, sh);
   }
   unsigned long long incl = sum;
 #pragma unroll
@@ -232,8 +286,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(
     const long long idx = base + t;
     if (idx <= ncells) {
       const unsigned long long e = thr + ex[t];
-      off[idx] = static_cast<int>(e & 0x7fffffffull);
-      item_off[idx] = static_cast<int>((e >> 31) & 0x7fffffffull);
+      if (off) off[idx] = static_cast<int>(e & 0x7fffffffull);
+      second_off[idx] = static_cast<int>((e >> 31) & 0x7fffffffull);
     }
   }
 }
@@ -248,100 +302,210 @@ __global__ __launch_bounds__(256) void k_emit_items(
   if (g >= ncells) return;
   const int c = count[g];
   if (c == 0) return;
-  const int nch = (c + 63) >> 6;
+  const int nch = (c + (1 << kChunkShift) - 1) >> kChunkShift;
   const int b = item_off[g];
   for (int q = 0; q < nch; ++q) items[b + q] = static_cast<int>(g);
 }
 
 __global__ __launch_bounds__(256) void k_scatter(
-    const int2* __restrict__ cellslot, const long long* __restrict__ fpt,
+    const int4* __restrict__ cellslot, const long long* __restrict__ fpt,
     int F, long long ntot, const int* __restrict__ off,
-    int* __restrict__ tmp_idx) {
+    long long* __restrict__ tmp_key) {
   const long long p = blockIdx.x * static_cast<long long>(blockDim.x) +
                       threadIdx.x;
   if (p >= ntot) return;
-  const int2 cs = cellslot[p];
+  const int4 cs = cellslot[p];
   if (cs.x < 0) return;
   const int f = find_frame(fpt, F, p);
-  tmp_idx[off[cs.x] + cs.y] = static_cast<int>(p - fpt[f]);
+  const long long i = p - fpt[f];
+  tmp_key[off[cs.x] + cs.y] = (static_cast<long long>(cs.z) << 32) | i;
 }
 
-// One wave per (cell, 64-occupant chunk). Each lane ranks its occupant
-// among the whole segment by original index, so segments come out in
-// ascending index order: the reference linked list walks a cell in
-// decreasing index order (src/knearest.rs:423-428), which the exact
-// walk replays by reading a segment backwards.
+// One wave per (cell, 64-member chunk). Each lane ranks its member within
+// the segment twice: by (Morton key, id) for the walk layout, whose
+// consecutive members are spatially close, and by id alone for byidx[],
+// through which the exact-order walk visits a cell in decreasing index
+// order as the reference linked list does (src/knearest.rs:423-428).
 __global__ __launch_bounds__(256) void k_cell_sort(
     const int* __restrict__ items, const int* __restrict__ item_off,
     long long ncells, const int* __restrict__ off,
-    const int* __restrict__ tmp_idx, const FrameDesc* __restrict__ fr,
+    const long long* __restrict__ tmp_key, const FrameDesc* __restrict__ fr,
     const long long* __restrict__ fcb, int F, double* __restrict__ sx,
     double* __restrict__ sy, double* __restrict__ sz, int* __restrict__ sidx,
-    int* __restrict__ scell) {
-  const int w = __builtin_amdgcn_readfirstlane(
-      static_cast<int>(blockIdx.x * (blockDim.x / kWave) +
-                       threadIdx.x / kWave));
+    int* __restrict__ scell, int* __restrict__ byidx) {
+  const int w = uni(static_cast<int>(blockIdx.x * (blockDim.x / kWave) +
+                                     threadIdx.x / kWave));
   const int lane = threadIdx.x & (kWave - 1);
   const int total = item_off[ncells];
   if (w >= total) return;
-  const int g = __builtin_amdgcn_readfirstlane(items[w]);
-  const int q = w - __builtin_amdgcn_readfirstlane(item_off[g]);
-  const int o0 = __builtin_amdgcn_readfirstlane(off[g]);
-  const int o1 = __builtin_amdgcn_readfirstlane(off[g + 1]);
+  const int g = uni(items[w]);
+  const int q = w - uni(item_off[g]);
+  const int o0 = uni(off[g]);
+  const int o1 = uni(off[g + 1]);
   const int a = o0 + q * kWave + lane;
   const bool valid = a < o1;
-  const int mine = valid ? tmp_idx[a] : INT_MAX;
-  int rank = 0;
-  for (int b = o0; b < o1; ++b) rank += tmp_idx[b] < mine ? 1 : 0;
+  const long long mine = valid ? tmp_key[a] : LLONG_MAX;
+  const int mid = static_cast<int>(mine & 0xffffffffll);
+  int rk = 0, ri = 0;
+  for (int b = o0; b < o1; ++b) {
+    const long long v = tmp_key[b];
+    rk += v < mine ? 1 : 0;
+    ri += static_cast<int>(v & 0xffffffffll) < mid ? 1 : 0;
+  }
   if (!valid) return;
   const int f = find_frame(fcb, F, g);
   const FrameDesc& d = fr[f];
   const CellGeom gm = geom_of(d);
-  const double* r = d.xyz + 3 * static_cast<size_t>(mine);
+  const double* r = d.xyz + 3 * static_cast<size_t>(mid);
   double s0, s1, s2, fx, fy, fz;
   fold_point(gm, r[0], r[1], r[2], s0, s1, s2, fx, fy, fz);
-  const int dst = o0 + rank;
+  const int dst = o0 + rk;
   sx[dst] = fx;
   sy[dst] = fy;
   sz[dst] = fz;
-  sidx[dst] = mine;
+  sidx[dst] = mid;
   scell[dst] = g;
+  byidx[o0 + ri] = dst;
 }
 
-// ----------------------------------------------------------- fast walk
-
-// Canonical order on (d2 bits, index). d2 >= 0 or +inf for finite input,
-// so the IEEE bit pattern orders like the value.
-__device__ __forceinline__ bool key_gt(long long da, int ja, long long db,
-                                       int jb) {
-  return da > db || (da == db && ja > jb);
+// Exact bounding boxes. One wave per 64-member chunk = four 16-point
+// tiles; 16-lane groups reduce min / max with shuffles. Boxes are stored
+// SoA: [lo_x, hi_x, lo_y, hi_y, lo_z, hi_z] x cap.
+__device__ __forceinline__ double red16_min(double v) {
+  for (int d = 8; d >= 1; d >>= 1) v = fmin(v, __shfl_xor(v, d, kWave));
+  return v;
+}
+__device__ __forceinline__ double red16_max(double v) {
+  for (int d = 8; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, kWave));
+  return v;
 }
 
-// Register heap of KC slots kept in DEScending canonical order. Slots
-// [0, k) start as (+inf, kEmptyJ); slots [k, KC) hold kUnusedBits and
-// never move. Slot 0 is the current k-th best, so the filter threshold
-// sits at a fixed register.
+__global__ __launch_bounds__(256) void k_tile_box(
+    const int* __restrict__ items, const int* __restrict__ item_off,
+    long long ncells, const int* __restrict__ off,
+    const int* __restrict__ tile_off, const double* __restrict__ sx,
+    const double* __restrict__ sy, const double* __restrict__ sz,
+    double* __restrict__ tbox, long long tcap) {
+  const int w = uni(static_cast<int>(blockIdx.x * (blockDim.x / kWave) +
+                                     threadIdx.x / kWave));
+  const int lane = threadIdx.x & (kWave - 1);
+  const int total = item_off[ncells];
+  if (w >= total) return;
+  const int g = uni(items[w]);
+  const int q = w - uni(item_off[g]);
+  const int o0 = uni(off[g]);
+  const int o1 = uni(off[g + 1]);
+  const int a = o0 + q * kWave + lane;
+  const bool valid = a < o1;
+  const double inf = as_f64(kInfBits);
+  const double x = valid ? sx[a] : 0.0, y = valid ? sy[a] : 0.0,
+               z = valid ? sz[a] : 0.0;
+  const double lx = red16_min(valid ? x : inf), hx = red16_max(valid ? x : -inf);
+  const double ly = red16_min(valid ? y : inf), hy = red16_max(valid ? y : -inf);
+  const double lz = red16_min(valid ? z : inf), hz = red16_max(valid ? z : -inf);
+  if ((lane & (kTile - 1)) == 0 && valid) {
+    const long long t = tile_off[g] + (q * kWave + lane) / kTile;
+    tbox[0 * tcap + t] = lx;
+    tbox[1 * tcap + t] = hx;
+    tbox[2 * tcap + t] = ly;
+    tbox[3 * tcap + t] = hy;
+    tbox[4 * tcap + t] = lz;
+    tbox[5 * tcap + t] = hz;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cell_box(
+    const int* __restrict__ off, const int* __restrict__ tile_off,
+    long long ncells, const double* __restrict__ tbox, long long tcap,
+    double* __restrict__ cbox) {
+  const long long g = blockIdx.x * static_cast<long long>(blockDim.x) +
+                      threadIdx.x;
+  if (g >= ncells) return;
+  const int t0 = tile_off[g], t1 = tile_off[g + 1];
+  if (t1 - t0 < 2) return;  // single-tile cells use the tile box
+  double b[6];
+  for (int c = 0; c < 6; ++c) b[c] = tbox[c * tcap + t0];
+  for (int t = t0 + 1; t < t1; ++t) {
+    for (int c = 0; c < 6; c += 2) {
+      b[c] = fmin(b[c], tbox[c * tcap + t]);
+      b[c + 1] = fmax(b[c + 1], tbox[(c + 1) * tcap + t]);
+    }
+  }
+  for (int c = 0; c < 6; ++c) cbox[c * ncells + g] = b[c];
+}
+
+// ------------------------------------------------------------ heap
+
+// Register heap of KC slots in descending d2 order. Slots [0, k) start as
+// (+inf, kEmptyJ); slots [k, KC) hold kUnusedBits and never move. Slot 0
+// is the current k-th best. Only d2 orders the heap: equal distances are
+// resolved afterwards (an equal pair inside the final k is reordered by
+// index at output, an equal pair across the k-th defers the source to the
+// exact-order walk).
 template <int KC>
 __device__ __forceinline__ void heap_insert(long long (&D)[KC], int (&J)[KC],
                                             long long xd, int xj,
                                             double& min_rej) {
-  if (!key_gt(D[0], J[0], xd, xj)) {
-    min_rej = fmin(min_rej, __longlong_as_double(xd));
+  if (!(D[0] > xd)) {
+    min_rej = fmin(min_rej, as_f64(xd));
     return;
   }
-  if (J[0] != kEmptyJ) min_rej = fmin(min_rej, __longlong_as_double(D[0]));
+  if (J[0] != kEmptyJ) min_rej = fmin(min_rej, as_f64(D[0]));
   bool gt[KC];
 #pragma unroll
-  for (int t = 0; t < KC; ++t) gt[t] = key_gt(D[t], J[t], xd, xj);
+  for (int t = 0; t < KC; ++t) gt[t] = D[t] > xd;
 #pragma unroll
   for (int t = 0; t < KC - 1; ++t) {
-    const long long nd = gt[t + 1] ? D[t + 1] : (gt[t] ? xd : D[t]);
-    const int nj = gt[t + 1] ? J[t + 1] : (gt[t] ? xj : J[t]);
-    D[t] = nd;
-    J[t] = nj;
+    const long long a = gt[t] ? xd : D[t];
+    const int aj = gt[t] ? xj : J[t];
+    D[t] = gt[t + 1] ? D[t + 1] : a;
+    J[t] = gt[t + 1] ? J[t + 1] : aj;
   }
   D[KC - 1] = gt[KC - 1] ? xd : D[KC - 1];
   J[KC - 1] = gt[KC - 1] ? xj : J[KC - 1];
+}
+
+// Finish a source: the row is exact unless a rejected or evicted d2
+// equals the k-th (a boundary tie, where the reference keeps whichever
+// index it met first). Equal d2 inside the row is ordered by index as
+// KHeap::finish does (src/knearest.rs:146-158). Returns false to defer.
+template <int KC>
+__device__ __forceinline__ bool heap_emit(long long (&D)[KC], int (&J)[KC],
+                                          int k, double min_rej,
+                                          int* out_nn, double* out_d2,
+                                          size_t row) {
+  if (J[0] != kEmptyJ && min_rej == as_f64(D[0])) return false;
+  // ties inside the row: order equal-d2 runs by index (odd-even passes
+  // over the few entries involved)
+  bool tie = false;
+#pragma unroll
+  for (int t = 0; t + 1 < KC; ++t) {
+    tie |= (t + 1 < k) && J[t] != kEmptyJ && D[t] == D[t + 1];
+  }
+  if (tie) {
+#pragma unroll 1
+    for (int pass = 0; pass < KC; ++pass) {
+#pragma unroll
+      for (int t = 0; t + 1 < KC; ++t) {
+        // descending storage: larger index first within equal d2
+        const bool sw = (t + 1 < k) && J[t] != kEmptyJ && D[t] == D[t + 1] &&
+                        J[t] < J[t + 1];
+        const int a = J[t];
+        J[t] = sw ? J[t + 1] : a;
+        J[t + 1] = sw ? a : J[t + 1];
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < KC; ++u) {
+    if (u < k) {
+      const int t = k - 1 - u;
+      const bool real = J[u] != kEmptyJ;
+      out_nn[row + t] = real ? J[u] : -1;
+      if (out_d2) out_d2[row + t] = real ? as_f64(D[u]) : nan_value();
+    }
+  }
+  return true;
 }
 
 __device__ __forceinline__ void defer_source(int s, int* defer_list,
@@ -350,42 +514,284 @@ __device__ __forceinline__ void defer_source(int s, int* defer_list,
   defer_list[pos] = s;
 }
 
-// One lane per active source in cell order (neighbouring lanes share
-// home or adjacent cells, so their candidate loads hit the same lines).
-// The result equals the reference's whenever no candidate outside the
-// final k ties the k-th distance: KHeap::push keeps, per index, the min
-// over visited images and the k smallest of those (src/knearest.rs:106-129),
-// which is order independent except for that tie. Ties are detected
-// exactly (min over rejected and evicted d2 equals the k-th d2) and
-// deferred, as are sources still unfinished at r_safe.
+// Lower bound of (q + shift - p)^2 over a box [lo, hi] of folded points,
+// evaluated with the walk's own rounded operations. Rounding is monotone,
+// so for every q in the box fl(fl(q + s) - p) lies between the rounded
+// corner differences and the bound never exceeds the d2 the walk would
+// compute: pruning on it cannot change any result.
+__device__ __forceinline__ double box_lb(double px, double py, double pz,
+                                         double lx, double hx, double ly,
+                                         double hy, double lz, double hz,
+                                         double shx, double shy, double shz,
+                                         bool zero) {
+  if (!zero) {
+    lx = lx + shx;
+    hx = hx + shx;
+    ly = ly + shy;
+    hy = hy + shy;
+    lz = lz + shz;
+    hz = hz + shz;
+  }
+  const double gx = fmax(fmax(lx - px, px - hx), 0.0);
+  const double gy = fmax(fmax(ly - py, py - hy), 0.0);
+  const double gz = fmax(fmax(lz - pz, pz - hz), 0.0);
+  return gx * gx + gy * gy + gz * gz;
+}
+
+struct WalkArgs {
+  const FrameDesc* fr;
+  const long long* fcb;
+  int F;
+  const int* fflags;
+  const double* sx;
+  const double* sy;
+  const double* sz;
+  const int* sidx;
+  const int* scell;
+  const int* off;
+  const int* tile_off;
+  const int* items;
+  const int* item_off;
+  long long ncells;
+  const double* tbox;
+  long long tcap;
+  const double* cbox;
+  int dense_min;
+  int* defer_list;
+  int* defer_count;
+};
+
+// idx-th cell of Chebyshev shell r: r == 1 walks the near-first table,
+// r > 1 the (2r+1)^3 cube lexicographically, skipping the interior.
+// Returns false for interior cells.
+__device__ __forceinline__ bool shell_offset(int r, int idx, int& dx, int& dy,
+                                             int& dz) {
+  if (r == 1) {
+    dx = kNear27[idx][0];
+    dy = kNear27[idx][1];
+    dz = kNear27[idx][2];
+    return true;
+  }
+  const int side = 2 * r + 1;
+  dx = idx / (side * side) - r;
+  dy = (idx / side) % side - r;
+  dz = idx % side - r;
+  return dx == -r || dx == r || dy == -r || dy == r || dz == -r || dz == r;
+}
+
+// ---------------------------------------------------------- dense walk
+
+// One wave = up to 64 sources of one home cell (sources in Morton order,
+// so the wave covers a compact part of the cell). Per candidate cell and
+// tile, each lane tests the tile's exact lower bound against its k-th
+// best; the wave skips the tile unless some lane needs it. Candidates are
+// wave-uniform (scalar loads); a lane that accepts one appends it to its
+// LDS buffer, and full buffers are merged into the register heaps.
 template <int KC>
-__global__ __launch_bounds__(256) void k_walk_fast(
-    const FrameDesc* __restrict__ fr, const long long* __restrict__ fcb,
-    int F, const int* __restrict__ fflags, const double* __restrict__ sx,
-    const double* __restrict__ sy, const double* __restrict__ sz,
-    const int* __restrict__ sidx, const int* __restrict__ scell,
-    const int* __restrict__ off, long long ncells, int* __restrict__ defer_list,
-    int* __restrict__ defer_count) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nact = off[ncells];
-  if (s >= nact) return;
-  const int g = scell[s];
-  const int f = find_frame(fcb, F, g);
-  const FrameDesc& d = fr[f];
+__global__ __launch_bounds__(kDenseWaves* kWave) void k_walk_dense(
+    WalkArgs A) {
+  __shared__ long long s_bd[kDenseWaves][kBuf][kWave];
+  __shared__ int s_bj[kDenseWaves][kBuf][kWave];
+  const int wv = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = uni(static_cast<int>(blockIdx.x * kDenseWaves + wv));
+  const int total = uni(A.item_off[A.ncells]);
+  if (w >= total) return;
+  const int g = uni(A.items[w]);
+  const int o0 = uni(A.off[g]);
+  const int cnt = uni(A.off[g + 1]) - o0;
+  if (cnt < A.dense_min) return;
+  const int q = w - uni(A.item_off[g]);
+  const int f = uni(find_frame(A.fcb, A.F, g));
+  const FrameDesc& d = A.fr[f];
   if (d.kc != KC) return;
-  if ((fflags[f] & kFrameNonFinite) != 0 || d.r_safe < 1) {
-    defer_source(s, defer_list, defer_count);
+  const int ls = q * kWave + lane;
+  const bool valid = ls < cnt;
+  const int s = o0 + (valid ? ls : 0);
+  if ((A.fflags[f] & kFrameNonFinite) != 0 || d.r_safe < 1) {
+    if (valid) defer_source(s, A.defer_list, A.defer_count);
     return;
   }
   const int k = d.k;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const long long cb = d.cell_base;
-  const int c = static_cast<int>(g - cb);
+  const int c = g - static_cast<int>(cb);
   const int hx = c % nx;
   const int hy = (c / nx) % ny;
   const int hz = c / (nx * ny);
-  const double px = sx[s], py = sy[s], pz = sz[s];
-  const int me = sidx[s];
+  const double px = A.sx[s], py = A.sy[s], pz = A.sz[s];
+  const int me = A.sidx[s];
+  const int ortho = d.ortho;
+  const double w0 = d.widths[0], w1 = d.widths[1], w2 = d.widths[2];
+
+  long long D[KC];
+  int J[KC];
+#pragma unroll
+  for (int t = 0; t < KC; ++t) {
+    D[t] = t < k ? kInfBits : kUnusedBits;
+    J[t] = kEmptyJ;
+  }
+  double min_rej = as_f64(kInfBits);
+  double thr = as_f64(kInfBits);
+  bool live = valid;
+  int nbuf = 0;
+  int done_reach = 0;
+  long long* bd = &s_bd[wv][0][lane];
+  int* bj = &s_bj[wv][0][lane];
+
+  auto flush = [&]() {
+    #pragma unroll 1
+    for (int e = 0; e < kBuf; ++e) {
+      if (!__ballot(e < nbuf)) break;
+      if (e < nbuf) {
+        heap_insert<KC>(D, J, bd[e * kWave], bj[e * kWave], min_rej);
+      }
+    }
+    nbuf = 0;
+    thr = as_f64(D[0]);
+  };
+
+  // Visit one candidate cell (unwrapped offset dx, dy, dz from home).
+  auto visit = [&](int dx, int dy, int dz, bool home) {
+    const int jx = hx + dx, jy = hy + dy, jz = hz + dz;
+    const int cx = jx < 0 ? jx + nx : (jx >= nx ? jx - nx : jx);
+    const int cy = jy < 0 ? jy + ny : (jy >= ny ? jy - ny : jy);
+    const int cz = jz < 0 ? jz + nz : (jz >= nz ? jz - nz : jz);
+    const int na = jx < 0 ? -1 : (jx >= nx ? 1 : 0);
+    const int nb = jy < 0 ? -1 : (jy >= ny ? 1 : 0);
+    const int nc = jz < 0 ? -1 : (jz >= nz ? 1 : 0);
+    const int gq = uni(static_cast<int>(cb) + (cz * ny + cy) * nx + cx);
+    const int a0 = uni(A.off[gq]);
+    const int n2 = uni(A.off[gq + 1]) - a0;
+    if (n2 == 0) return;
+    const bool zero = (na | nb | nc) == 0;
+    double shx = 0.0, shy = 0.0, shz = 0.0;
+    if (!zero) {
+      if (ortho) {
+        shx = static_cast<double>(na) * w0;
+        shy = static_cast<double>(nb) * w1;
+        shz = static_cast<double>(nc) * w2;
+      } else {
+        const double fa = na, fb = nb, fc = nc;
+        shx = fa * d.h[0] + fb * d.h[3] + fc * d.h[6];
+        shy = fa * d.h[1] + fb * d.h[4] + fc * d.h[7];
+        shz = fa * d.h[2] + fb * d.h[5] + fc * d.h[8];
+      }
+    }
+    const int nt = (n2 + kTile - 1) >> kTileShift;
+    if (nt > 1) {
+      const long long nc6 = A.ncells;
+      const double lb = box_lb(px, py, pz, A.cbox[0 * nc6 + gq],
+                               A.cbox[1 * nc6 + gq], A.cbox[2 * nc6 + gq],
+                               A.cbox[3 * nc6 + gq], A.cbox[4 * nc6 + gq],
+                               A.cbox[5 * nc6 + gq], shx, shy, shz, zero);
+      if (!__ballot(live && lb <= thr)) return;
+    }
+    const int t0 = uni(A.tile_off[gq]);
+    const int start = home ? (q * (kWave / kTile)) % nt : 0;
+    #pragma unroll 1
+    for (int tt = 0; tt < nt; ++tt) {
+      int t = start + tt;
+      t = t >= nt ? t - nt : t;
+      const long long ti = t0 + t;
+      const double lb = box_lb(px, py, pz, A.tbox[0 * A.tcap + ti],
+                               A.tbox[1 * A.tcap + ti], A.tbox[2 * A.tcap + ti],
+                               A.tbox[3 * A.tcap + ti], A.tbox[4 * A.tcap + ti],
+                               A.tbox[5 * A.tcap + ti], shx, shy, shz, zero);
+      const bool need = live && lb <= thr;
+      if (!__ballot(need)) continue;
+      const int base = a0 + t * kTile;
+      const int e = min(kTile, n2 - t * kTile);
+      #pragma unroll 1
+      for (int h = 0; h < e; h += 8) {
+        const int lim = e - h;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          if (u < lim) {
+            const int pos = base + h + u;
+            double qx = A.sx[pos], qy = A.sy[pos], qz = A.sz[pos];
+            const int j = A.sidx[pos];
+            if (!zero) {
+              qx = qx + shx;
+              qy = qy + shy;
+              qz = qz + shz;
+            }
+            const double ddx = qx - px, ddy = qy - py, ddz = qz - pz;
+            const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            const bool acc = need && d2 <= thr && !(home && j == me);
+            if (acc) {
+              bd[nbuf * kWave] = __double_as_longlong(d2);
+              bj[nbuf * kWave] = j;
+              nbuf += 1;
+            }
+          }
+        }
+        if (__ballot(nbuf > kBuf - 8)) flush();
+      }
+    }
+  };
+
+  #pragma unroll 1
+  for (int r = 1; r <= d.r_safe; ++r) {
+    const int side = 2 * r + 1;
+    const int ncube = side * side * side;
+    #pragma unroll 1
+    for (int idx = 0; idx < ncube; ++idx) {
+      int dx, dy, dz;
+      if (!shell_offset(r, idx, dx, dy, dz)) continue;
+      visit(dx, dy, dz, r == 1 && idx == 0);
+    }
+    flush();
+    // stop test, src/knearest.rs:466-471
+    if (live && J[0] != kEmptyJ) {
+      const double bound = static_cast<double>(r) * d.cell_min;
+      if (as_f64(D[0]) <= bound * bound) {
+        live = false;
+        done_reach = r;
+      }
+    }
+    if (!__ballot(live)) break;
+  }
+  if (!valid) return;
+  const size_t row = static_cast<size_t>(me) * k;
+  if (done_reach == 0 ||
+      !heap_emit<KC>(D, J, k, min_rej, d.out_nn, d.out_d2, row)) {
+    defer_source(s, A.defer_list, A.defer_count);
+    return;
+  }
+  if (d.out_reach) d.out_reach[me] = done_reach;
+}
+
+// --------------------------------------------------------- sparse walk
+
+// One lane per source of a cell below the dense threshold, in cell
+// order. Each lane walks its own stencil; per lane the exact tile bounds
+// skip tiles that cannot beat its k-th best.
+template <int KC>
+__global__ __launch_bounds__(256) void k_walk_sparse(WalkArgs A) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nact = A.off[A.ncells];
+  if (s >= nact) return;
+  const int g = A.scell[s];
+  const int o0 = A.off[g];
+  const int cnt = A.off[g + 1] - o0;
+  if (cnt >= A.dense_min) return;
+  const int f = find_frame(A.fcb, A.F, g);
+  const FrameDesc& d = A.fr[f];
+  if (d.kc != KC) return;
+  if ((A.fflags[f] & kFrameNonFinite) != 0 || d.r_safe < 1) {
+    defer_source(s, A.defer_list, A.defer_count);
+    return;
+  }
+  const int k = d.k;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const long long cb = d.cell_base;
+  const int c = g - static_cast<int>(cb);
+  const int hx = c % nx;
+  const int hy = (c / nx) % ny;
+  const int hz = c / (nx * ny);
+  const double px = A.sx[s], py = A.sy[s], pz = A.sz[s];
+  const int me = A.sidx[s];
   const int ortho = d.ortho;
   const double w0 = d.widths[0], w1 = d.widths[1], w2 = d.widths[2];
   double hm[9];
@@ -399,72 +805,82 @@ __global__ __launch_bounds__(256) void k_walk_fast(
     D[t] = t < k ? kInfBits : kUnusedBits;
     J[t] = kEmptyJ;
   }
-  double min_rej = __longlong_as_double(kInfBits);
+  double min_rej = as_f64(kInfBits);
   int done_reach = 0;
-  for (int r = 1; r <= d.r_safe; ++r) {
-    for (int dx = -r; dx <= r; ++dx) {
-      const int jx = hx + dx;
-      const int cx = jx < 0 ? jx + nx : (jx >= nx ? jx - nx : jx);
-      const int na = jx < 0 ? -1 : (jx >= nx ? 1 : 0);
-      for (int dy = -r; dy <= r; ++dy) {
-        const int jy = hy + dy;
-        const int cy = jy < 0 ? jy + ny : (jy >= ny ? jy - ny : jy);
-        const int nb = jy < 0 ? -1 : (jy >= ny ? 1 : 0);
-        const bool face = r == 1 || dx == -r || dx == r || dy == -r || dy == r;
-        const int step = face ? 1 : 2 * r;
-        for (int dz = -r; dz <= r; dz += step) {
-          const int jz = hz + dz;
-          const int cz = jz < 0 ? jz + nz : (jz >= nz ? jz - nz : jz);
-          const int nc = jz < 0 ? -1 : (jz >= nz ? 1 : 0);
-          double shx, shy, shz;
-          if (ortho) {
-            shx = static_cast<double>(na) * w0;
-            shy = static_cast<double>(nb) * w1;
-            shz = static_cast<double>(nc) * w2;
-          } else {
-            const double fa = na, fb = nb, fc = nc;
-            shx = fa * hm[0] + fb * hm[3] + fc * hm[6];
-            shy = fa * hm[1] + fb * hm[4] + fc * hm[7];
-            shz = fa * hm[2] + fb * hm[5] + fc * hm[8];
-          }
-          const long long gq = cb + (static_cast<long long>(cz) * ny + cy) * nx + cx;
-          const int a0 = off[gq];
-          const int a1 = off[gq + 1];
-          for (int t = a0; t < a1; ++t) {
-            const int j = sidx[t];
-            const double d2 = dist2_shifted(px, py, pz, sx[t], sy[t], sz[t],
-                                            shx, shy, shz);
-            if (j != me && d2 <= __longlong_as_double(D[0])) {
-              heap_insert<KC>(D, J, __double_as_longlong(d2), j, min_rej);
-            }
-          }
+
+  auto visit = [&](int dx, int dy, int dz) {
+    const int jx = hx + dx, jy = hy + dy, jz = hz + dz;
+    const int cx = jx < 0 ? jx + nx : (jx >= nx ? jx - nx : jx);
+    const int cy = jy < 0 ? jy + ny : (jy >= ny ? jy - ny : jy);
+    const int cz = jz < 0 ? jz + nz : (jz >= nz ? jz - nz : jz);
+    const int na = jx < 0 ? -1 : (jx >= nx ? 1 : 0);
+    const int nb = jy < 0 ? -1 : (jy >= ny ? 1 : 0);
+    const int nc = jz < 0 ? -1 : (jz >= nz ? 1 : 0);
+    const int gq = static_cast<int>(cb) + (cz * ny + cy) * nx + cx;
+    const int a0 = A.off[gq];
+    const int n2 = A.off[gq + 1] - a0;
+    if (n2 == 0) return;
+    const bool zero = (na | nb | nc) == 0;
+    double shx = 0.0, shy = 0.0, shz = 0.0;
+    if (!zero) {
+      if (ortho) {
+        shx = static_cast<double>(na) * w0;
+        shy = static_cast<double>(nb) * w1;
+        shz = static_cast<double>(nc) * w2;
+      } else {
+        const double fa = na, fb = nb, fc = nc;
+        shx = fa * hm[0] + fb * hm[3] + fc * hm[6];
+        shy = fa * hm[1] + fb * hm[4] + fc * hm[7];
+        shz = fa * hm[2] + fb * hm[5] + fc * hm[8];
+      }
+    }
+    const int t0 = A.tile_off[gq];
+    const int nt = (n2 + kTile - 1) >> kTileShift;
+    #pragma unroll 1
+    for (int t = 0; t < nt; ++t) {
+      const long long ti = t0 + t;
+      const double lb = box_lb(px, py, pz, A.tbox[0 * A.tcap + ti],
+                               A.tbox[1 * A.tcap + ti], A.tbox[2 * A.tcap + ti],
+                               A.tbox[3 * A.tcap + ti], A.tbox[4 * A.tcap + ti],
+                               A.tbox[5 * A.tcap + ti], shx, shy, shz, zero);
+      if (lb > as_f64(D[0])) continue;
+      const int b = a0 + t * kTile;
+      const int e = b + min(kTile, n2 - t * kTile);
+      #pragma unroll 1
+      for (int pos = b; pos < e; ++pos) {
+        const int j = A.sidx[pos];
+        const double d2 = dist2_shifted(px, py, pz, A.sx[pos], A.sy[pos],
+                                        A.sz[pos], shx, shy, shz);
+        if (j != me && d2 <= as_f64(D[0])) {
+          heap_insert<KC>(D, J, __double_as_longlong(d2), j, min_rej);
         }
       }
     }
-    // stop test, src/knearest.rs:466-471
+  };
+
+  #pragma unroll 1
+  for (int r = 1; r <= d.r_safe; ++r) {
+    const int side = 2 * r + 1;
+    const int ncube = side * side * side;
+    #pragma unroll 1
+    for (int idx = 0; idx < ncube; ++idx) {
+      int dx, dy, dz;
+      if (!shell_offset(r, idx, dx, dy, dz)) continue;
+      visit(dx, dy, dz);
+    }
     if (J[0] != kEmptyJ) {
       const double bound = static_cast<double>(r) * d.cell_min;
-      if (__longlong_as_double(D[0]) <= bound * bound) {
+      if (as_f64(D[0]) <= bound * bound) {
         done_reach = r;
         break;
       }
     }
   }
-  if (done_reach == 0 || min_rej == __longlong_as_double(D[0])) {
-    defer_source(s, defer_list, defer_count);
-    return;
-  }
   const size_t row = static_cast<size_t>(me) * k;
-#pragma unroll
-  for (int u = 0; u < KC; ++u) {
-    if (u < k) {
-      const int t = k - 1 - u;
-      const bool real = J[u] != kEmptyJ;
-      d.out_nn[row + t] = real ? J[u] : -1;
-      if (d.out_d2) {
-        d.out_d2[row + t] = real ? __longlong_as_double(D[u]) : nan_value();
-      }
-    }
+  if (done_reach == 0 ||
+      !heap_emit<KC>(D, J, k, min_rej, d.out_nn, d.out_d2, row)) {
+    defer_source(s, A.defer_list, A.defer_count);
+    return;
   }
   if (d.out_reach) d.out_reach[me] = done_reach;
 }
@@ -493,7 +909,7 @@ __device__ __forceinline__ long long total_key(double x) {
 
 // Exact-order replay of the reference walk for deferred sources: the
 // lexicographic stencil (src/knearest.rs:435-465), each cell in
-// decreasing index order, KHeap::push / worst / finish
+// decreasing index order (through byidx), KHeap::push / worst / finish
 // (src/knearest.rs:106-158) with the heap in a strided global scratch.
 // One lane per source; a grid-stride loop over the deferred list.
 __global__ __launch_bounds__(64) void k_walk_exact(
@@ -501,8 +917,9 @@ __global__ __launch_bounds__(64) void k_walk_exact(
     int F, const double* __restrict__ sx, const double* __restrict__ sy,
     const double* __restrict__ sz, const int* __restrict__ sidx,
     const int* __restrict__ scell, const int* __restrict__ off,
-    const int* __restrict__ defer_list, const int* __restrict__ defer_count,
-    double* __restrict__ hd2, int* __restrict__ hidx) {
+    const int* __restrict__ byidx, const int* __restrict__ defer_list,
+    const int* __restrict__ defer_count, double* __restrict__ hd2,
+    int* __restrict__ hidx) {
   const long long tid = blockIdx.x * static_cast<long long>(blockDim.x) +
                         threadIdx.x;
   const long long nthreads = static_cast<long long>(gridDim.x) * blockDim.x;
@@ -543,10 +960,11 @@ __global__ __launch_bounds__(64) void k_walk_exact(
             const int a0 = off[cb + cq];
             const int a1 = off[cb + cq + 1];
             for (int t = a1 - 1; t >= a0; --t) {
-              const int j = sidx[t];
+              const int pos = byidx[t];
+              const int j = sidx[pos];
               if (j == me) continue;
-              const double d2 = dist2_shifted(px, py, pz, sx[t], sy[t], sz[t],
-                                              shx, shy, shz);
+              const double d2 = dist2_shifted(px, py, pz, sx[pos], sy[pos],
+                                              sz[pos], shx, shy, shz);
               // KHeap::push. With a full heap, d2 >= the worst slot can
               // change nothing: an index already present holds at most
               // that value, and the replacement needs d2 < worst.
@@ -642,6 +1060,12 @@ int fast_bucket(int k) {
 
 Engine::Engine() {
   check(hipGetDevice(&device_), "hipGetDevice");
+  // tuning override, in the spirit of the reference's LINKCELL_TPP /
+  // LINKCELL_BLOCK (src/gpu/knearest.cpp:948-960)
+  if (const char* v = std::getenv("LINKCELL_DENSE_MIN")) {
+    const int x = std::atoi(v);
+    if (x >= 1) dense_min_ = x;
+  }
 }
 
 Engine::~Engine() {
@@ -684,10 +1108,11 @@ void Engine::grow(Buf& b, size_t bytes) {
 }
 
 void Engine::free_all() {
-  Buf* all[] = {&frames_, &fpt_,     &fcb_,      &count_,  &status_,
-                &counters_, &off_,   &item_off_, &items_,  &cellslot_,
-                &tmp_idx_, &sx_,     &sy_,       &sz_,     &sidx_,
-                &scell_,   &defer_,  &heap_d2_,  &heap_idx_, &flags_};
+  Buf* all[] = {&frames_,   &fpt_,      &fcb_,    &count_,  &status_,
+                &counters_, &off_,      &tile_off_, &item_off_, &items_,
+                &cellslot_, &tmp_key_,  &sx_,     &sy_,     &sz_,
+                &sidx_,     &scell_,    &byidx_,  &tbox_,   &cbox_,
+                &defer_,    &heap_d2_,  &heap_idx_, &flags_};
   for (Buf* b : all) {
     if (b->p != nullptr) (void)hipFree(b->p);
     b->p = nullptr;
@@ -764,27 +1189,34 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   }
   fpt[static_cast<size_t>(F)] = ntot;
   fcb[static_cast<size_t>(F)] = ctot;
-  if (ntot >= INT_MAX || ctot >= INT_MAX) {
-    throw std::runtime_error("batch exceeds 2^31 - 1 points or cells");
+  if (ntot >= INT_MAX / 2 || ctot >= INT_MAX / 2) {
+    throw std::runtime_error("batch exceeds 2^30 points or cells");
   }
   const long long tiles = ctot / kScanTile + 1;
+  // 16-point tiles: at most one partial tile per non-empty cell
+  const long long tcap = ntot / kTile + std::min(ntot, ctot) + 1;
+  const long long max_items = std::min<long long>(ntot, ctot + ntot / kWave + 1) + 1;
 
   grow(frames_, sizeof(FrameDesc) * static_cast<size_t>(F));
   grow(fpt_, sizeof(long long) * (static_cast<size_t>(F) + 1));
   grow(fcb_, sizeof(long long) * (static_cast<size_t>(F) + 1));
   grow(count_, sizeof(int) * static_cast<size_t>(ctot));
-  grow(status_, sizeof(unsigned long long) * static_cast<size_t>(tiles));
+  grow(status_, sizeof(unsigned long long) * 2 * static_cast<size_t>(tiles));
   grow(counters_, sizeof(int) * 4);
   grow(off_, sizeof(int) * (static_cast<size_t>(ctot) + 1));
+  grow(tile_off_, sizeof(int) * (static_cast<size_t>(ctot) + 1));
   grow(item_off_, sizeof(int) * (static_cast<size_t>(ctot) + 1));
-  grow(items_, sizeof(int) * (static_cast<size_t>(ntot) + 1));
-  grow(cellslot_, sizeof(int2) * static_cast<size_t>(ntot));
-  grow(tmp_idx_, sizeof(int) * static_cast<size_t>(ntot));
-  grow(sx_, sizeof(double) * static_cast<size_t>(ntot));
-  grow(sy_, sizeof(double) * static_cast<size_t>(ntot));
-  grow(sz_, sizeof(double) * static_cast<size_t>(ntot));
-  grow(sidx_, sizeof(int) * static_cast<size_t>(ntot));
+  grow(items_, sizeof(int) * static_cast<size_t>(max_items));
+  grow(cellslot_, sizeof(int4) * static_cast<size_t>(ntot));
+  grow(tmp_key_, sizeof(long long) * static_cast<size_t>(ntot));
+  grow(sx_, sizeof(double) * static_cast<size_t>(ntot + kPad));
+  grow(sy_, sizeof(double) * static_cast<size_t>(ntot + kPad));
+  grow(sz_, sizeof(double) * static_cast<size_t>(ntot + kPad));
+  grow(sidx_, sizeof(int) * static_cast<size_t>(ntot + kPad));
   grow(scell_, sizeof(int) * static_cast<size_t>(ntot));
+  grow(byidx_, sizeof(int) * static_cast<size_t>(ntot));
+  grow(tbox_, sizeof(double) * 6 * static_cast<size_t>(tcap));
+  grow(cbox_, sizeof(double) * 6 * static_cast<size_t>(ctot));
   grow(defer_, sizeof(int) * static_cast<size_t>(ntot));
   grow(flags_, sizeof(int) * static_cast<size_t>(F));
   // exact-walk scratch: one heap of kmax slots per resident lane
@@ -846,8 +1278,8 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   // captured into a graph)
   k_zero_u32<<<blocks_for(ctot, 256), 256, 0, st>>>(as<unsigned>(count_.p),
                                                     ctot);
-  k_zero_u32<<<blocks_for(tiles * 2, 256), 256, 0, st>>>(
-      as<unsigned>(status_.p), tiles * 2);
+  k_zero_u32<<<blocks_for(tiles * 4, 256), 256, 0, st>>>(
+      as<unsigned>(status_.p), tiles * 4);
   k_zero_u32<<<1, 64, 0, st>>>(as<unsigned>(counters_.p), 4);
   k_zero_u32<<<blocks_for(F, 256), 256, 0, st>>>(as<unsigned>(flags_.p), F);
 
@@ -855,45 +1287,75 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   const long long* dfpt = as<long long>(fpt_.p);
   const long long* dfcb = as<long long>(fcb_.p);
   int* counters = as<int>(counters_.p);
+  unsigned long long* status = as<unsigned long long>(status_.p);
 
   k_fold_count<<<blocks_for(ntot, 256), 256, 0, st>>>(
-      dfr, dfpt, F, ntot, as<int>(count_.p), as<int2>(cellslot_.p),
+      dfr, dfpt, F, ntot, as<int>(count_.p), as<int4>(cellslot_.p),
       as<int>(flags_.p));
   check(hipGetLastError(), "fold_count launch");
   if (timing_) check(hipEventRecord(ev_[1], st), "event record");
 
   k_scan<<<static_cast<unsigned>(tiles), kScanThreads, 0, st>>>(
-      as<int>(count_.p), ctot, as<int>(off_.p), as<int>(item_off_.p),
-      as<unsigned long long>(status_.p), counters + 0);
+      as<int>(count_.p), ctot, kTileShift, as<int>(off_.p),
+      as<int>(tile_off_.p), status, counters + 0);
+  k_scan<<<static_cast<unsigned>(tiles), kScanThreads, 0, st>>>(
+      as<int>(count_.p), ctot, kChunkShift, nullptr, as<int>(item_off_.p),
+      status + tiles, counters + 2);
   check(hipGetLastError(), "scan launch");
   if (timing_) check(hipEventRecord(ev_[2], st), "event record");
 
   k_emit_items<<<blocks_for(ctot, 256), 256, 0, st>>>(
       as<int>(count_.p), as<int>(item_off_.p), ctot, as<int>(items_.p));
   k_scatter<<<blocks_for(ntot, 256), 256, 0, st>>>(
-      as<int2>(cellslot_.p), dfpt, F, ntot, as<int>(off_.p),
-      as<int>(tmp_idx_.p));
+      as<int4>(cellslot_.p), dfpt, F, ntot, as<int>(off_.p),
+      as<long long>(tmp_key_.p));
   check(hipGetLastError(), "scatter launch");
   if (timing_) check(hipEventRecord(ev_[3], st), "event record");
 
-  // items <= ntot (one chunk per non-empty cell at most per 64 points,
-  // plus one per cell); launch for the bound and exit early
-  const long long max_items = std::min<long long>(ntot, ctot + ntot / 64 + 1);
-  k_cell_sort<<<blocks_for(max_items * kWave, 256), 256, 0, st>>>(
+  const unsigned item_blocks = blocks_for(max_items * kWave, 256);
+  k_cell_sort<<<item_blocks, 256, 0, st>>>(
       as<int>(items_.p), as<int>(item_off_.p), ctot, as<int>(off_.p),
-      as<int>(tmp_idx_.p), dfr, dfcb, F, as<double>(sx_.p), as<double>(sy_.p),
-      as<double>(sz_.p), as<int>(sidx_.p), as<int>(scell_.p));
+      as<long long>(tmp_key_.p), dfr, dfcb, F, as<double>(sx_.p),
+      as<double>(sy_.p), as<double>(sz_.p), as<int>(sidx_.p),
+      as<int>(scell_.p), as<int>(byidx_.p));
+  k_tile_box<<<item_blocks, 256, 0, st>>>(
+      as<int>(items_.p), as<int>(item_off_.p), ctot, as<int>(off_.p),
+      as<int>(tile_off_.p), as<double>(sx_.p), as<double>(sy_.p),
+      as<double>(sz_.p), as<double>(tbox_.p), tcap);
+  k_cell_box<<<blocks_for(ctot, 256), 256, 0, st>>>(
+      as<int>(off_.p), as<int>(tile_off_.p), ctot, as<double>(tbox_.p), tcap,
+      as<double>(cbox_.p));
   check(hipGetLastError(), "cell_sort launch");
   if (timing_) check(hipEventRecord(ev_[4], st), "event record");
 
-  const unsigned wblocks = blocks_for(ntot, 256);
+  WalkArgs wa;
+  wa.fr = dfr;
+  wa.fcb = dfcb;
+  wa.F = F;
+  wa.fflags = as<int>(flags_.p);
+  wa.sx = as<double>(sx_.p);
+  wa.sy = as<double>(sy_.p);
+  wa.sz = as<double>(sz_.p);
+  wa.sidx = as<int>(sidx_.p);
+  wa.scell = as<int>(scell_.p);
+  wa.off = as<int>(off_.p);
+  wa.tile_off = as<int>(tile_off_.p);
+  wa.items = as<int>(items_.p);
+  wa.item_off = as<int>(item_off_.p);
+  wa.ncells = ctot;
+  wa.tbox = as<double>(tbox_.p);
+  wa.tcap = tcap;
+  wa.cbox = as<double>(cbox_.p);
+  wa.dense_min = dense_min_;
+  wa.defer_list = as<int>(defer_.p);
+  wa.defer_count = counters + 1;
+  const unsigned dense_blocks =
+      static_cast<unsigned>((max_items + kDenseWaves - 1) / kDenseWaves);
+  const unsigned sparse_blocks = blocks_for(ntot, 256);
 #define LC_WALK(KC)                                                         \
   if (any_fast[KC]) {                                                       \
-    k_walk_fast<KC><<<wblocks, 256, 0, st>>>(                               \
-        dfr, dfcb, F, as<int>(flags_.p), as<double>(sx_.p),                 \
-        as<double>(sy_.p), as<double>(sz_.p), as<int>(sidx_.p),             \
-        as<int>(scell_.p), as<int>(off_.p), ctot, as<int>(defer_.p),        \
-        counters + 1);                                                      \
+    k_walk_dense<KC><<<dense_blocks, kDenseWaves * kWave, 0, st>>>(wa);     \
+    k_walk_sparse<KC><<<sparse_blocks, 256, 0, st>>>(wa);                   \
     check(hipGetLastError(), "walk launch");                                \
   }
   LC_WALK(4)
@@ -907,16 +1369,17 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
     any_exact_frame |= desc[static_cast<size_t>(f)].kc == 0;
   }
   if (any_exact_frame) {
-    k_route_exact<<<wblocks, 256, 0, st>>>(dfr, dfcb, F, as<int>(scell_.p),
-                                           as<int>(off_.p), ctot,
-                                           as<int>(defer_.p), counters + 1);
+    k_route_exact<<<sparse_blocks, 256, 0, st>>>(
+        dfr, dfcb, F, as<int>(scell_.p), as<int>(off_.p), ctot,
+        as<int>(defer_.p), counters + 1);
     check(hipGetLastError(), "route launch");
   }
   if (timing_) check(hipEventRecord(ev_[5], st), "event record");
   k_walk_exact<<<static_cast<unsigned>(exact_threads / 64), 64, 0, st>>>(
       dfr, dfcb, F, as<double>(sx_.p), as<double>(sy_.p), as<double>(sz_.p),
-      as<int>(sidx_.p), as<int>(scell_.p), as<int>(off_.p), as<int>(defer_.p),
-      counters + 1, as<double>(heap_d2_.p), as<int>(heap_idx_.p));
+      as<int>(sidx_.p), as<int>(scell_.p), as<int>(off_.p), as<int>(byidx_.p),
+      as<int>(defer_.p), counters + 1, as<double>(heap_d2_.p),
+      as<int>(heap_idx_.p));
   check(hipGetLastError(), "exact walk launch");
   if (timing_) {
     check(hipEventRecord(ev_[6], st), "event record");

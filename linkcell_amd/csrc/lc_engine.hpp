@@ -81,6 +81,10 @@ class Engine {
   void set_timing(bool on) { timing_ = on; }
   const Timings& last_timings();
   void set_mode(WalkMode m) { mode_ = m; }
+  // Home cells with at least this many sources take the wave-per-cell
+  // walk, the rest the lane-per-source walk.
+  void set_dense_min(int v) { dense_min_ = v < 1 ? 1 : v; }
+  int dense_min() const { return dense_min_; }
 
  private:
   struct Buf {
@@ -92,9 +96,9 @@ class Engine {
 
   hipStream_t stream_ = nullptr;
   int device_ = -1;
-  Buf frames_, fpt_, fcb_, count_, status_, counters_, off_, item_off_,
-      items_, cellslot_, tmp_idx_, sx_, sy_, sz_, sidx_, scell_, defer_,
-      heap_d2_, heap_idx_, flags_;
+  Buf frames_, fpt_, fcb_, count_, status_, counters_, off_, tile_off_,
+      item_off_, items_, cellslot_, tmp_key_, sx_, sy_, sz_, sidx_, scell_,
+      byidx_, tbox_, cbox_, defer_, heap_d2_, heap_idx_, flags_;
   void* staging_ = nullptr;
   size_t staging_bytes_ = 0;
   hipEvent_t staging_ev_ = nullptr;
@@ -102,6 +106,7 @@ class Engine {
   bool timing_ = false;
   bool timings_pending_ = false;
   WalkMode mode_ = WalkMode::kAuto;
+  int dense_min_ = 32;
   hipEvent_t ev_[8] = {};
   Timings timings_{};
 };
