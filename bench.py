@@ -189,6 +189,10 @@ def main():
         kt.append(ws.last_timing())
     ws.set_timing(False)
     kern = {key: float(np.mean([t[key] for t in kt])) for key in kt[0]}
+    ws.set_stats(True)
+    step(wait=True)
+    work = ws.last_stats()
+    ws.set_stats(False)
     n_act = int((reach > 0).sum()) if mask is None else int(mask.sum())
     cid, plan = fold_cells(xyz, cell, hint)
     bwalk, n_act_b = walk_bytes(cid, plan, reach, k)
@@ -214,6 +218,7 @@ def main():
         "walk_ms": kern["walk"] + kern["exact"],
         "walk_qps": n_act / t_walk,
         "kernels_ms": kern,
+        "walk_work": work,
         "reach_hist": np.bincount(reach).tolist(),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -37,6 +37,20 @@ int lc_gpu_set_mode(lc_gpu_workspace *ws, int mode);
 int lc_gpu_set_timing(lc_gpu_workspace *ws, int on);
 int lc_gpu_last_timing(lc_gpu_workspace *ws, double *ms7, int *deferred);
 
+/* Home-cell occupancy from which the wave-per-cell walk handles a cell
+ * (the rest take the lane-per-source walk). Default 32, or the
+ * LINKCELL_DENSE_MIN environment variable. */
+int lc_gpu_set_dense_min(lc_gpu_workspace *ws, int dense_min);
+
+/* Device work counters of the walk kernels (diagnostics). When on, each
+ * search accumulates them; lc_gpu_last_stats waits for the last search
+ * and copies up to n counters: dense waves, dense tile tests, dense tiles
+ * walked, dense candidate steps, dense heap merge steps, dense appends,
+ * sparse sources, sparse tile tests, sparse tiles walked, sparse
+ * candidate evaluations, sparse heap inserts. */
+int lc_gpu_set_stats(lc_gpu_workspace *ws, int on);
+int lc_gpu_last_stats(lc_gpu_workspace *ws, unsigned long long *out, int n);
+
 #ifdef __cplusplus
 }
 #endif

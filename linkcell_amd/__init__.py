@@ -36,7 +36,12 @@ def _lib_handle():
     return _lib.load()
 
 
-__version__ = _lib_handle().lc_version().decode()
+def __getattr__(name):
+    # Resolved on first use so that importing the package (for example to
+    # run linkcell_amd.build) never loads a stale or missing library.
+    if name == "__version__":
+        return _lib_handle().lc_version().decode()
+    raise AttributeError(name)
 
 
 def gpu_available() -> bool:
@@ -130,6 +135,23 @@ class Workspace:
         out = {k: float(v) for k, v in zip(keys, ms)}
         out["deferred"] = int(deferred.value)
         return out
+
+    def set_dense_min(self, v: int):
+        self._check(self._lib.lc_gpu_set_dense_min(self._ws, int(v)))
+
+    def set_stats(self, on: bool):
+        self._check(self._lib.lc_gpu_set_stats(self._ws, 1 if on else 0))
+
+    STAT_NAMES = ("dense_waves", "dense_tile_tests", "dense_tiles", "dense_steps",
+                  "dense_merges", "dense_appends", "sparse_sources",
+                  "sparse_tile_tests", "sparse_tiles", "sparse_evals",
+                  "sparse_inserts")
+
+    def last_stats(self) -> dict:
+        n = len(self.STAT_NAMES)
+        buf = (C.c_ulonglong * n)()
+        self._check(self._lib.lc_gpu_last_stats(self._ws, buf, n))
+        return {k: int(v) for k, v in zip(self.STAT_NAMES, buf)}
 
     def knearest_ptr(self, xyz_ptr: int, n: int, n_frames: int, cell12, k: int,
                      out_nn_ptr: int, out_d2_ptr: int = 0, mask_ptr: int = 0,

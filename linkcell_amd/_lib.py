@@ -22,7 +22,8 @@ EXPORTS = (
     "lc_gpu_alloc", "lc_gpu_free", "lc_gpu_fill_i32", "lc_gpu_memcpy",
     # include/linkcell_amd.h
     "lc_gpu_knearest_ex", "lc_gpu_set_mode", "lc_gpu_set_timing",
-    "lc_gpu_last_timing",
+    "lc_gpu_last_timing", "lc_gpu_set_dense_min", "lc_gpu_set_stats",
+    "lc_gpu_last_stats",
 )
 
 
@@ -71,6 +72,9 @@ _PROTOS = {
     "lc_gpu_set_mode": (C.c_int, [_vp, C.c_int]),
     "lc_gpu_set_timing": (C.c_int, [_vp, C.c_int]),
     "lc_gpu_last_timing": (C.c_int, [_vp, _dp, _ip]),
+    "lc_gpu_set_dense_min": (C.c_int, [_vp, C.c_int]),
+    "lc_gpu_set_stats": (C.c_int, [_vp, C.c_int]),
+    "lc_gpu_last_stats": (C.c_int, [_vp, C.POINTER(C.c_ulonglong), C.c_int]),
 }
 
 _lib = None

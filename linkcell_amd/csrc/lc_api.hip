@@ -361,6 +361,34 @@ int lc_gpu_last_timing(lc_gpu_workspace* ws, double* ms7, int* deferred) {
   });
 }
 
+int lc_gpu_set_dense_min(lc_gpu_workspace* ws, int dense_min) {
+  if (ws == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  ws->ws.impl()->engine.set_dense_min(dense_min);
+  set_gpu_err(nullptr);
+  return 0;
+}
+
+int lc_gpu_set_stats(lc_gpu_workspace* ws, int on) {
+  if (ws == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  ws->ws.impl()->engine.set_stats(on != 0);
+  set_gpu_err(nullptr);
+  return 0;
+}
+
+int lc_gpu_last_stats(lc_gpu_workspace* ws, unsigned long long* out, int n) {
+  if (ws == nullptr || out == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] { ws->ws.impl()->engine.read_stats(out, n); });
+}
+
 int lc_gpu_alloc(void** ptr, size_t bytes) {
   if (ptr == nullptr) {
     set_gpu_err("null pointer");

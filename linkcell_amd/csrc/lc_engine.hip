@@ -559,7 +559,29 @@ struct WalkArgs {
   int dense_min;
   int* defer_list;
   int* defer_count;
+  // optional work counters (null = off): see kStat* indices
+  unsigned long long* stats;
 };
+
+enum {
+  kStatDenseWaves = 0,
+  kStatDenseTileTests,
+  kStatDenseTiles,
+  kStatDenseSteps,
+  kStatDenseMerges,
+  kStatDenseAppends,
+  kStatSparseSources,
+  kStatSparseTileTests,
+  kStatSparseTiles,
+  kStatSparseEvals,
+  kStatSparseInserts,
+  kStatCount = 16
+};
+
+__device__ __forceinline__ void stat_add(unsigned long long* st, int i,
+                                         unsigned long long v) {
+  if (st != nullptr && v != 0) atomicAdd(&st[i], v);
+}
 
 // idx-th cell of Chebyshev shell r: r == 1 walks the near-first table,
 // r > 1 the (2r+1)^3 cube lexicographically, skipping the interior.
@@ -638,11 +660,14 @@ __global__ __launch_bounds__(kDenseWaves* kWave) void k_walk_dense(
   int done_reach = 0;
   long long* bd = &s_bd[wv][0][lane];
   int* bj = &s_bj[wv][0][lane];
+  unsigned st_tests = 0, st_tiles = 0, st_steps = 0, st_merges = 0,
+           st_appends = 0;
 
   auto flush = [&]() {
     #pragma unroll 1
     for (int e = 0; e < kBuf; ++e) {
       if (!__ballot(e < nbuf)) break;
+      st_merges += 1;
       if (e < nbuf) {
         heap_insert<KC>(D, J, bd[e * kWave], bj[e * kWave], min_rej);
       }
@@ -699,12 +724,15 @@ __global__ __launch_bounds__(kDenseWaves* kWave) void k_walk_dense(
                                A.tbox[3 * A.tcap + ti], A.tbox[4 * A.tcap + ti],
                                A.tbox[5 * A.tcap + ti], shx, shy, shz, zero);
       const bool need = live && lb <= thr;
+      st_tests += 1;
       if (!__ballot(need)) continue;
+      st_tiles += 1;
       const int base = a0 + t * kTile;
       const int e = min(kTile, n2 - t * kTile);
       #pragma unroll 1
       for (int h = 0; h < e; h += 8) {
         const int lim = e - h;
+        st_steps += lim < 8 ? lim : 8;
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           if (u < lim) {
@@ -723,6 +751,7 @@ __global__ __launch_bounds__(kDenseWaves* kWave) void k_walk_dense(
               bd[nbuf * kWave] = __double_as_longlong(d2);
               bj[nbuf * kWave] = j;
               nbuf += 1;
+              st_appends += 1;
             }
           }
         }
@@ -751,6 +780,18 @@ __global__ __launch_bounds__(kDenseWaves* kWave) void k_walk_dense(
       }
     }
     if (!__ballot(live)) break;
+  }
+  if (A.stats != nullptr) {
+    unsigned long long app = st_appends;
+    for (int dd = 32; dd >= 1; dd >>= 1) app += __shfl_xor(app, dd, kWave);
+    if (lane == 0) {
+      stat_add(A.stats, kStatDenseWaves, 1);
+      stat_add(A.stats, kStatDenseTileTests, st_tests);
+      stat_add(A.stats, kStatDenseTiles, st_tiles);
+      stat_add(A.stats, kStatDenseSteps, st_steps);
+      stat_add(A.stats, kStatDenseMerges, st_merges);
+      stat_add(A.stats, kStatDenseAppends, app);
+    }
   }
   if (!valid) return;
   const size_t row = static_cast<size_t>(me) * k;
@@ -807,6 +848,7 @@ __global__ __launch_bounds__(256) void k_walk_sparse(WalkArgs A) {
   }
   double min_rej = as_f64(kInfBits);
   int done_reach = 0;
+  unsigned st_tests = 0, st_tiles = 0, st_evals = 0, st_ins = 0;
 
   auto visit = [&](int dx, int dy, int dz) {
     const int jx = hx + dx, jy = hy + dy, jz = hz + dz;
@@ -843,7 +885,9 @@ __global__ __launch_bounds__(256) void k_walk_sparse(WalkArgs A) {
                                A.tbox[1 * A.tcap + ti], A.tbox[2 * A.tcap + ti],
                                A.tbox[3 * A.tcap + ti], A.tbox[4 * A.tcap + ti],
                                A.tbox[5 * A.tcap + ti], shx, shy, shz, zero);
+      st_tests += 1;
       if (lb > as_f64(D[0])) continue;
+      st_tiles += 1;
       const int b = a0 + t * kTile;
       const int e = b + min(kTile, n2 - t * kTile);
       #pragma unroll 1
@@ -851,7 +895,9 @@ __global__ __launch_bounds__(256) void k_walk_sparse(WalkArgs A) {
         const int j = A.sidx[pos];
         const double d2 = dist2_shifted(px, py, pz, A.sx[pos], A.sy[pos],
                                         A.sz[pos], shx, shy, shz);
+        st_evals += 1;
         if (j != me && d2 <= as_f64(D[0])) {
+          st_ins += 1;
           heap_insert<KC>(D, J, __double_as_longlong(d2), j, min_rej);
         }
       }
@@ -875,6 +921,13 @@ __global__ __launch_bounds__(256) void k_walk_sparse(WalkArgs A) {
         break;
       }
     }
+  }
+  if (A.stats != nullptr) {
+    stat_add(A.stats, kStatSparseSources, 1);
+    stat_add(A.stats, kStatSparseTileTests, st_tests);
+    stat_add(A.stats, kStatSparseTiles, st_tiles);
+    stat_add(A.stats, kStatSparseEvals, st_evals);
+    stat_add(A.stats, kStatSparseInserts, st_ins);
   }
   const size_t row = static_cast<size_t>(me) * k;
   if (done_reach == 0 ||
@@ -1050,6 +1103,17 @@ unsigned blocks_for(long long n, int threads) {
 
 }  // namespace
 
+void Engine::read_stats(unsigned long long* out, int n) {
+  sync();
+  std::vector<unsigned long long> h(kStatCount, 0);
+  if (stats_.p != nullptr) {
+    check(hipMemcpy(h.data(), stats_.p, sizeof(unsigned long long) * kStatCount,
+                    hipMemcpyDeviceToHost),
+          "DtoH stats");
+  }
+  for (int i = 0; i < n && i < kStatCount; ++i) out[i] = h[static_cast<size_t>(i)];
+}
+
 int fast_bucket(int k) {
   if (k <= 4) return 4;
   if (k <= 8) return 8;
@@ -1112,7 +1176,7 @@ void Engine::free_all() {
                 &counters_, &off_,      &tile_off_, &item_off_, &items_,
                 &cellslot_, &tmp_key_,  &sx_,     &sy_,     &sz_,
                 &sidx_,     &scell_,    &byidx_,  &tbox_,   &cbox_,
-                &defer_,    &heap_d2_,  &heap_idx_, &flags_};
+                &defer_,    &heap_d2_,  &heap_idx_, &flags_, &stats_};
   for (Buf* b : all) {
     if (b->p != nullptr) (void)hipFree(b->p);
     b->p = nullptr;
@@ -1349,6 +1413,12 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   wa.dense_min = dense_min_;
   wa.defer_list = as<int>(defer_.p);
   wa.defer_count = counters + 1;
+  wa.stats = nullptr;
+  if (stats_on_) {
+    grow(stats_, sizeof(unsigned long long) * kStatCount);
+    k_zero_u32<<<1, 64, 0, st>>>(as<unsigned>(stats_.p), 2 * kStatCount);
+    wa.stats = as<unsigned long long>(stats_.p);
+  }
   const unsigned dense_blocks =
       static_cast<unsigned>((max_items + kDenseWaves - 1) / kDenseWaves);
   const unsigned sparse_blocks = blocks_for(ntot, 256);

@@ -85,6 +85,9 @@ class Engine {
   // walk, the rest the lane-per-source walk.
   void set_dense_min(int v) { dense_min_ = v < 1 ? 1 : v; }
   int dense_min() const { return dense_min_; }
+  // Device work counters of the walks (diagnostics).
+  void set_stats(bool on) { stats_on_ = on; }
+  void read_stats(unsigned long long* out, int n);
 
  private:
   struct Buf {
@@ -98,7 +101,7 @@ class Engine {
   int device_ = -1;
   Buf frames_, fpt_, fcb_, count_, status_, counters_, off_, tile_off_,
       item_off_, items_, cellslot_, tmp_key_, sx_, sy_, sz_, sidx_, scell_,
-      byidx_, tbox_, cbox_, defer_, heap_d2_, heap_idx_, flags_;
+      byidx_, tbox_, cbox_, defer_, heap_d2_, heap_idx_, flags_, stats_;
   void* staging_ = nullptr;
   size_t staging_bytes_ = 0;
   hipEvent_t staging_ev_ = nullptr;
@@ -107,6 +110,7 @@ class Engine {
   bool timings_pending_ = false;
   WalkMode mode_ = WalkMode::kAuto;
   int dense_min_ = 32;
+  bool stats_on_ = false;
   hipEvent_t ev_[8] = {};
   Timings timings_{};
 };
