@@ -47,7 +47,7 @@ int lc_gpu_set_dense_min(lc_gpu_workspace *ws, int dense_min);
  * and copies up to n counters: dense waves, dense tile tests, dense tiles
  * walked, dense candidate steps, dense heap merge steps, dense appends,
  * sparse sources, sparse tile tests, sparse tiles walked, sparse
- * candidate evaluations, sparse heap inserts. */
+ * candidate evaluations, sparse batch merges. */
 int lc_gpu_set_stats(lc_gpu_workspace *ws, int on);
 int lc_gpu_last_stats(lc_gpu_workspace *ws, unsigned long long *out, int n);
 

@@ -145,7 +145,7 @@ class Workspace:
     STAT_NAMES = ("dense_waves", "dense_tile_tests", "dense_tiles", "dense_steps",
                   "dense_merges", "dense_appends", "sparse_sources",
                   "sparse_tile_tests", "sparse_tiles", "sparse_evals",
-                  "sparse_inserts")
+                  "sparse_merges")
 
     def last_stats(self) -> dict:
         n = len(self.STAT_NAMES)

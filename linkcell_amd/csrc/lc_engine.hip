@@ -60,8 +60,6 @@ constexpr unsigned long long kNanBits = 0x7ff8000000000000ull;
 constexpr int kTile = 16;      // points per bounding-box tile
 constexpr int kTileShift = 4;
 constexpr int kChunkShift = 6;  // sources per dense work item (one wave)
-constexpr int kBuf = 16;        // per-lane LDS candidate buffer entries
-constexpr int kDenseWaves = 4;  // waves per dense-walk block
 constexpr int kPad = 16;        // slack elements after SoA arrays (group reads)
 
 // r = 1 stencil in near-first order: home, 6 faces, 12 edges, 8 corners.
@@ -436,73 +434,129 @@ __global__ __launch_bounds__(256) void k_cell_box(
 
 // ------------------------------------------------------------ heap
 
-// Register heap of KC slots in descending d2 order. Slots [0, k) start as
-// (+inf, kEmptyJ); slots [k, KC) hold kUnusedBits and never move. Slot 0
-// is the current k-th best. Only d2 orders the heap: equal distances are
-// resolved afterwards (an equal pair inside the final k is reordered by
-// index at output, an equal pair across the k-th defers the source to the
+// Per-lane top-k in registers, ascending by d2 bits (d2 >= 0 or +inf for
+// finite input, so the IEEE bit pattern orders like the value). Slots
+// [0, KC - k) hold kUnusedBits, below every key, and never leave; slots
+// [KC - k, KC) are the k best so far, empty ones (+inf, kEmptyJ), so
+// d[KC - 1] is the k-th best. Only d2 orders the heap: equal distances are
+// resolved at the end (an equal pair inside the final k is ordered by
+// index, an equal pair across the k-th defers the source to the
 // exact-order walk).
 template <int KC>
-__device__ __forceinline__ void heap_insert(long long (&D)[KC], int (&J)[KC],
-                                            long long xd, int xj,
-                                            double& min_rej) {
-  if (!(D[0] > xd)) {
-    min_rej = fmin(min_rej, as_f64(xd));
-    return;
-  }
-  if (J[0] != kEmptyJ) min_rej = fmin(min_rej, as_f64(D[0]));
-  bool gt[KC];
+struct Top {
+  long long d[KC];
+  int j[KC];
+};
+
+template <int KC>
+__device__ __forceinline__ void top_init(Top<KC>& h, int k) {
 #pragma unroll
-  for (int t = 0; t < KC; ++t) gt[t] = D[t] > xd;
-#pragma unroll
-  for (int t = 0; t < KC - 1; ++t) {
-    const long long a = gt[t] ? xd : D[t];
-    const int aj = gt[t] ? xj : J[t];
-    D[t] = gt[t + 1] ? D[t + 1] : a;
-    J[t] = gt[t + 1] ? J[t + 1] : aj;
+  for (int t = 0; t < KC; ++t) {
+    h.d[t] = t < KC - k ? kUnusedBits : kInfBits;
+    h.j[t] = kEmptyJ;
   }
-  D[KC - 1] = gt[KC - 1] ? xd : D[KC - 1];
-  J[KC - 1] = gt[KC - 1] ? xj : J[KC - 1];
 }
 
-// Finish a source: the row is exact unless a rejected or evicted d2
-// equals the k-th (a boundary tie, where the reference keeps whichever
-// index it met first). Equal d2 inside the row is ordered by index as
-// KHeap::finish does (src/knearest.rs:146-158). Returns false to defer.
+__device__ __forceinline__ void cswap(long long& da, int& ja, long long& db,
+                                      int& jb, bool up) {
+  const bool sw = up ? (da > db) : (da < db);
+  const long long td = da;
+  const int tj = ja;
+  da = sw ? db : da;
+  ja = sw ? jb : ja;
+  db = sw ? td : db;
+  jb = sw ? tj : jb;
+}
+
 template <int KC>
-__device__ __forceinline__ bool heap_emit(long long (&D)[KC], int (&J)[KC],
-                                          int k, double min_rej,
-                                          int* out_nn, double* out_d2,
-                                          size_t row) {
-  if (J[0] != kEmptyJ && min_rej == as_f64(D[0])) return false;
-  // ties inside the row: order equal-d2 runs by index (odd-even passes
-  // over the few entries involved)
+__device__ __forceinline__ void bitonic_sort(long long (&d)[KC], int (&j)[KC]) {
+#pragma unroll
+  for (int size = 2; size <= KC; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+      for (int i = 0; i < KC; ++i) {
+        const int p = i ^ stride;
+        if (p > i) cswap(d[i], j[i], d[p], j[p], (i & size) == 0);
+      }
+    }
+  }
+}
+
+// Merge n buffered candidates (LDS column of this lane, stride kWave) into
+// the top-k: sort the batch, keep the KC smallest of (top, batch) with the
+// bitonic min trick (top ascending against batch descending), clean the
+// bitonic result. Real entries that drop out feed min_rej, the smallest d2
+// ever rejected, used for boundary-tie detection.
+template <int KC>
+__device__ __forceinline__ void batch_merge(Top<KC>& h, const long long* bd,
+                                            const int* bj, int n,
+                                            double& min_rej) {
+  long long b[KC];
+  int bx[KC];
+#pragma unroll
+  for (int t = 0; t < KC; ++t) {
+    const bool v = t < n;
+    b[t] = v ? bd[t * kWave] : kInfBits;
+    bx[t] = v ? bj[t * kWave] : kEmptyJ;
+  }
+  bitonic_sort<KC>(b, bx);
+#pragma unroll
+  for (int t = 0; t < KC; ++t) {
+    const long long x = h.d[t], y = b[KC - 1 - t];
+    const int xj = h.j[t], yj = bx[KC - 1 - t];
+    const bool ty = y < x;
+    h.d[t] = ty ? y : x;
+    h.j[t] = ty ? yj : xj;
+    const long long hi = ty ? x : y;
+    const int hij = ty ? xj : yj;
+    if (hij != kEmptyJ) min_rej = fmin(min_rej, as_f64(hi));
+  }
+#pragma unroll
+  for (int stride = KC >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+    for (int i = 0; i < KC; ++i) {
+      const int p = i ^ stride;
+      if (p > i) cswap(h.d[i], h.j[i], h.d[p], h.j[p], true);
+    }
+  }
+}
+
+// Finish a source with a full heap: the row is exact unless a rejected or
+// evicted d2 equals the k-th (a boundary tie, where the reference keeps
+// whichever index it met first). Equal d2 inside the row is ordered by
+// index as KHeap::finish does (src/knearest.rs:146-158). Returns false to
+// defer the source.
+template <int KC>
+__device__ __forceinline__ bool top_emit(Top<KC>& h, int k, double min_rej,
+                                         int* out_nn, double* out_d2,
+                                         size_t row) {
+  if (min_rej == as_f64(h.d[KC - 1])) return false;
   bool tie = false;
 #pragma unroll
   for (int t = 0; t + 1 < KC; ++t) {
-    tie |= (t + 1 < k) && J[t] != kEmptyJ && D[t] == D[t + 1];
+    tie |= (t >= KC - k) && h.d[t] == h.d[t + 1];
   }
   if (tie) {
 #pragma unroll 1
     for (int pass = 0; pass < KC; ++pass) {
 #pragma unroll
       for (int t = 0; t + 1 < KC; ++t) {
-        // descending storage: larger index first within equal d2
-        const bool sw = (t + 1 < k) && J[t] != kEmptyJ && D[t] == D[t + 1] &&
-                        J[t] < J[t + 1];
-        const int a = J[t];
-        J[t] = sw ? J[t + 1] : a;
-        J[t + 1] = sw ? a : J[t + 1];
+        const bool sw = (t >= KC - k) && h.d[t] == h.d[t + 1] &&
+                        h.j[t] > h.j[t + 1];
+        const int a = h.j[t];
+        h.j[t] = sw ? h.j[t + 1] : a;
+        h.j[t + 1] = sw ? a : h.j[t + 1];
       }
     }
   }
 #pragma unroll
-  for (int u = 0; u < KC; ++u) {
-    if (u < k) {
-      const int t = k - 1 - u;
-      const bool real = J[u] != kEmptyJ;
-      out_nn[row + t] = real ? J[u] : -1;
-      if (out_d2) out_d2[row + t] = real ? as_f64(D[u]) : nan_value();
+  for (int t = 0; t < KC; ++t) {
+    if (t >= KC - k) {
+      const int o = t - (KC - k);
+      const bool real = h.j[t] != kEmptyJ;
+      out_nn[row + o] = real ? h.j[t] : -1;
+      if (out_d2) out_d2[row + o] = real ? as_f64(h.d[t]) : nan_value();
     }
   }
   return true;
@@ -574,7 +628,7 @@ enum {
   kStatSparseTileTests,
   kStatSparseTiles,
   kStatSparseEvals,
-  kStatSparseInserts,
+  kStatSparseMerges,
   kStatCount = 16
 };
 
@@ -601,6 +655,14 @@ __device__ __forceinline__ bool shell_offset(int r, int idx, int& dx, int& dy,
   return dx == -r || dx == r || dy == -r || dy == r || dz == -r || dz == r;
 }
 
+// Waves per block of the fast walks: the per-lane LDS buffers hold KC
+// entries of 12 bytes, so the largest bucket runs two waves per block.
+template <int KC>
+struct WalkShape {
+  static constexpr int kWaves = KC >= 32 ? 2 : 4;
+  static constexpr int kGroup = KC >= 16 ? 8 : KC / 2;  // appends per check
+};
+
 // ---------------------------------------------------------- dense walk
 
 // One wave = up to 64 sources of one home cell (sources in Morton order,
@@ -608,15 +670,18 @@ __device__ __forceinline__ bool shell_offset(int r, int idx, int& dx, int& dy,
 // tile, each lane tests the tile's exact lower bound against its k-th
 // best; the wave skips the tile unless some lane needs it. Candidates are
 // wave-uniform (scalar loads); a lane that accepts one appends it to its
-// LDS buffer, and full buffers are merged into the register heaps.
+// LDS buffer, and the buffers are merged into the register top-k in
+// batches (all lanes at once, whenever one is nearly full).
 template <int KC>
-__global__ __launch_bounds__(kDenseWaves* kWave) void k_walk_dense(
+__global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_dense(
     WalkArgs A) {
-  __shared__ long long s_bd[kDenseWaves][kBuf][kWave];
-  __shared__ int s_bj[kDenseWaves][kBuf][kWave];
+  constexpr int W = WalkShape<KC>::kWaves;
+  constexpr int G = WalkShape<KC>::kGroup;
+  __shared__ long long s_bd[W][KC][kWave];
+  __shared__ int s_bj[W][KC][kWave];
   const int wv = threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
-  const int w = uni(static_cast<int>(blockIdx.x * kDenseWaves + wv));
+  const int w = uni(static_cast<int>(blockIdx.x * W + wv));
   const int total = uni(A.item_off[A.ncells]);
   if (w >= total) return;
   const int g = uni(A.items[w]);
@@ -646,13 +711,8 @@ __global__ __launch_bounds__(kDenseWaves* kWave) void k_walk_dense(
   const int ortho = d.ortho;
   const double w0 = d.widths[0], w1 = d.widths[1], w2 = d.widths[2];
 
-  long long D[KC];
-  int J[KC];
-#pragma unroll
-  for (int t = 0; t < KC; ++t) {
-    D[t] = t < k ? kInfBits : kUnusedBits;
-    J[t] = kEmptyJ;
-  }
+  Top<KC> h;
+  top_init<KC>(h, k);
   double min_rej = as_f64(kInfBits);
   double thr = as_f64(kInfBits);
   bool live = valid;
@@ -664,16 +724,10 @@ __global__ __launch_bounds__(kDenseWaves* kWave) void k_walk_dense(
            st_appends = 0;
 
   auto flush = [&]() {
-    #pragma unroll 1
-    for (int e = 0; e < kBuf; ++e) {
-      if (!__ballot(e < nbuf)) break;
-      st_merges += 1;
-      if (e < nbuf) {
-        heap_insert<KC>(D, J, bd[e * kWave], bj[e * kWave], min_rej);
-      }
-    }
+    st_merges += 1;
+    if (nbuf > 0) batch_merge<KC>(h, bd, bj, nbuf, min_rej);
     nbuf = 0;
-    thr = as_f64(D[0]);
+    thr = as_f64(h.d[KC - 1]);
   };
 
   // Visit one candidate cell (unwrapped offset dx, dy, dz from home).
@@ -714,7 +768,7 @@ __global__ __launch_bounds__(kDenseWaves* kWave) void k_walk_dense(
     }
     const int t0 = uni(A.tile_off[gq]);
     const int start = home ? (q * (kWave / kTile)) % nt : 0;
-    #pragma unroll 1
+#pragma unroll 1
     for (int tt = 0; tt < nt; ++tt) {
       int t = start + tt;
       t = t >= nt ? t - nt : t;
@@ -729,14 +783,14 @@ __global__ __launch_bounds__(kDenseWaves* kWave) void k_walk_dense(
       st_tiles += 1;
       const int base = a0 + t * kTile;
       const int e = min(kTile, n2 - t * kTile);
-      #pragma unroll 1
-      for (int h = 0; h < e; h += 8) {
-        const int lim = e - h;
-        st_steps += lim < 8 ? lim : 8;
+#pragma unroll 1
+      for (int hh = 0; hh < e; hh += G) {
+        const int lim = e - hh;
+        st_steps += lim < G ? lim : G;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < G; ++u) {
           if (u < lim) {
-            const int pos = base + h + u;
+            const int pos = base + hh + u;
             double qx = A.sx[pos], qy = A.sy[pos], qz = A.sz[pos];
             const int j = A.sidx[pos];
             if (!zero) {
@@ -755,26 +809,26 @@ __global__ __launch_bounds__(kDenseWaves* kWave) void k_walk_dense(
             }
           }
         }
-        if (__ballot(nbuf > kBuf - 8)) flush();
+        if (__ballot(nbuf > KC - G)) flush();
       }
     }
   };
 
-  #pragma unroll 1
+#pragma unroll 1
   for (int r = 1; r <= d.r_safe; ++r) {
     const int side = 2 * r + 1;
     const int ncube = side * side * side;
-    #pragma unroll 1
+#pragma unroll 1
     for (int idx = 0; idx < ncube; ++idx) {
       int dx, dy, dz;
       if (!shell_offset(r, idx, dx, dy, dz)) continue;
       visit(dx, dy, dz, r == 1 && idx == 0);
     }
-    flush();
+    if (__ballot(nbuf > 0)) flush();
     // stop test, src/knearest.rs:466-471
-    if (live && J[0] != kEmptyJ) {
+    if (live && h.j[KC - 1] != kEmptyJ) {
       const double bound = static_cast<double>(r) * d.cell_min;
-      if (as_f64(D[0]) <= bound * bound) {
+      if (as_f64(h.d[KC - 1]) <= bound * bound) {
         live = false;
         done_reach = r;
       }
@@ -796,7 +850,7 @@ __global__ __launch_bounds__(kDenseWaves* kWave) void k_walk_dense(
   if (!valid) return;
   const size_t row = static_cast<size_t>(me) * k;
   if (done_reach == 0 ||
-      !heap_emit<KC>(D, J, k, min_rej, d.out_nn, d.out_d2, row)) {
+      !top_emit<KC>(h, k, min_rej, d.out_nn, d.out_d2, row)) {
     defer_source(s, A.defer_list, A.defer_count);
     return;
   }
@@ -805,11 +859,22 @@ __global__ __launch_bounds__(kDenseWaves* kWave) void k_walk_dense(
 
 // --------------------------------------------------------- sparse walk
 
-// One lane per source of a cell below the dense threshold, in cell
-// order. Each lane walks its own stencil; per lane the exact tile bounds
-// skip tiles that cannot beat its k-th best.
+// One lane per source of a cell below the dense threshold, in cell order
+// (neighbouring lanes share home or adjacent cells). Each lane walks its
+// own stencil: the home cell from its own 16-point tile outwards, then
+// the near-first neighbour order; cells and tiles whose exact lower bound
+// exceeds the lane's k-th best are skipped. Accepted candidates go to the
+// lane's LDS buffer; whenever any lane's buffer is nearly full, every lane
+// with entries merges its batch.
 template <int KC>
-__global__ __launch_bounds__(256) void k_walk_sparse(WalkArgs A) {
+__global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
+    WalkArgs A) {
+  constexpr int W = WalkShape<KC>::kWaves;
+  constexpr int G = WalkShape<KC>::kGroup;
+  __shared__ long long s_bd[W][KC][kWave];
+  __shared__ int s_bj[W][KC][kWave];
+  const int wv = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int nact = A.off[A.ncells];
   if (s >= nact) return;
@@ -833,24 +898,33 @@ __global__ __launch_bounds__(256) void k_walk_sparse(WalkArgs A) {
   const int hz = c / (nx * ny);
   const double px = A.sx[s], py = A.sy[s], pz = A.sz[s];
   const int me = A.sidx[s];
+  const int own_tile = (s - o0) >> kTileShift;
   const int ortho = d.ortho;
   const double w0 = d.widths[0], w1 = d.widths[1], w2 = d.widths[2];
   double hm[9];
 #pragma unroll
   for (int t = 0; t < 9; ++t) hm[t] = d.h[t];
 
-  long long D[KC];
-  int J[KC];
-#pragma unroll
-  for (int t = 0; t < KC; ++t) {
-    D[t] = t < k ? kInfBits : kUnusedBits;
-    J[t] = kEmptyJ;
-  }
+  Top<KC> h;
+  top_init<KC>(h, k);
   double min_rej = as_f64(kInfBits);
+  double thr = as_f64(kInfBits);
+  int nbuf = 0;
   int done_reach = 0;
-  unsigned st_tests = 0, st_tiles = 0, st_evals = 0, st_ins = 0;
+  long long* bd = &s_bd[wv][0][lane];
+  int* bj = &s_bj[wv][0][lane];
+  unsigned st_tests = 0, st_tiles = 0, st_evals = 0, st_merges = 0;
 
-  auto visit = [&](int dx, int dy, int dz) {
+  auto flush = [&]() {
+    if (nbuf > 0) {
+      st_merges += 1;
+      batch_merge<KC>(h, bd, bj, nbuf, min_rej);
+      nbuf = 0;
+      thr = as_f64(h.d[KC - 1]);
+    }
+  };
+
+  auto visit = [&](int dx, int dy, int dz, bool home) {
     const int jx = hx + dx, jy = hy + dy, jz = hz + dz;
     const int cx = jx < 0 ? jx + nx : (jx >= nx ? jx - nx : jx);
     const int cy = jy < 0 ? jy + ny : (jy >= ny ? jy - ny : jy);
@@ -876,47 +950,85 @@ __global__ __launch_bounds__(256) void k_walk_sparse(WalkArgs A) {
         shz = fa * hm[2] + fb * hm[5] + fc * hm[8];
       }
     }
-    const int t0 = A.tile_off[gq];
     const int nt = (n2 + kTile - 1) >> kTileShift;
-    #pragma unroll 1
-    for (int t = 0; t < nt; ++t) {
+    if (nt > 1) {
+      const long long nc6 = A.ncells;
+      const double lb = box_lb(px, py, pz, A.cbox[0 * nc6 + gq],
+                               A.cbox[1 * nc6 + gq], A.cbox[2 * nc6 + gq],
+                               A.cbox[3 * nc6 + gq], A.cbox[4 * nc6 + gq],
+                               A.cbox[5 * nc6 + gq], shx, shy, shz, zero);
+      if (lb > thr) return;
+    }
+    const int t0 = A.tile_off[gq];
+    const int start = home ? own_tile : 0;
+#pragma unroll 1
+    for (int tt = 0; tt < nt; ++tt) {
+      int t = start + tt;
+      t = t >= nt ? t - nt : t;
       const long long ti = t0 + t;
-      const double lb = box_lb(px, py, pz, A.tbox[0 * A.tcap + ti],
-                               A.tbox[1 * A.tcap + ti], A.tbox[2 * A.tcap + ti],
-                               A.tbox[3 * A.tcap + ti], A.tbox[4 * A.tcap + ti],
-                               A.tbox[5 * A.tcap + ti], shx, shy, shz, zero);
-      st_tests += 1;
-      if (lb > as_f64(D[0])) continue;
+      if (nt > 1 || !home) {
+        const double lb = box_lb(px, py, pz, A.tbox[0 * A.tcap + ti],
+                                 A.tbox[1 * A.tcap + ti],
+                                 A.tbox[2 * A.tcap + ti],
+                                 A.tbox[3 * A.tcap + ti],
+                                 A.tbox[4 * A.tcap + ti],
+                                 A.tbox[5 * A.tcap + ti], shx, shy, shz, zero);
+        st_tests += 1;
+        if (lb > thr) continue;
+      }
       st_tiles += 1;
-      const int b = a0 + t * kTile;
-      const int e = b + min(kTile, n2 - t * kTile);
-      #pragma unroll 1
-      for (int pos = b; pos < e; ++pos) {
-        const int j = A.sidx[pos];
-        const double d2 = dist2_shifted(px, py, pz, A.sx[pos], A.sy[pos],
-                                        A.sz[pos], shx, shy, shz);
-        st_evals += 1;
-        if (j != me && d2 <= as_f64(D[0])) {
-          st_ins += 1;
-          heap_insert<KC>(D, J, __double_as_longlong(d2), j, min_rej);
+      const int base = a0 + t * kTile;
+      const int e = min(kTile, n2 - t * kTile);
+#pragma unroll 1
+      for (int hh = 0; hh < e; hh += G) {
+        const int lim = e - hh;
+        double qx[G], qy[G], qz[G];
+        int jj[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+          const int pos = base + hh + (u < lim ? u : 0);
+          qx[u] = A.sx[pos];
+          qy[u] = A.sy[pos];
+          qz[u] = A.sz[pos];
+          jj[u] = A.sidx[pos];
         }
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+          double ax = qx[u], ay = qy[u], az = qz[u];
+          if (!zero) {
+            ax = ax + shx;
+            ay = ay + shy;
+            az = az + shz;
+          }
+          const double ddx = ax - px, ddy = ay - py, ddz = az - pz;
+          const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+          const bool acc = u < lim && d2 <= thr && jj[u] != me;
+          if (acc) {
+            bd[nbuf * kWave] = __double_as_longlong(d2);
+            bj[nbuf * kWave] = jj[u];
+            nbuf += 1;
+          }
+        }
+        st_evals += lim < G ? lim : G;
+        if (__ballot(nbuf > KC - G)) flush();
       }
     }
   };
 
-  #pragma unroll 1
+#pragma unroll 1
   for (int r = 1; r <= d.r_safe; ++r) {
     const int side = 2 * r + 1;
     const int ncube = side * side * side;
-    #pragma unroll 1
+#pragma unroll 1
     for (int idx = 0; idx < ncube; ++idx) {
       int dx, dy, dz;
       if (!shell_offset(r, idx, dx, dy, dz)) continue;
-      visit(dx, dy, dz);
+      visit(dx, dy, dz, r == 1 && idx == 0);
     }
-    if (J[0] != kEmptyJ) {
+    flush();
+    if (h.j[KC - 1] != kEmptyJ) {
       const double bound = static_cast<double>(r) * d.cell_min;
-      if (as_f64(D[0]) <= bound * bound) {
+      if (as_f64(h.d[KC - 1]) <= bound * bound) {
         done_reach = r;
         break;
       }
@@ -927,11 +1039,11 @@ __global__ __launch_bounds__(256) void k_walk_sparse(WalkArgs A) {
     stat_add(A.stats, kStatSparseTileTests, st_tests);
     stat_add(A.stats, kStatSparseTiles, st_tiles);
     stat_add(A.stats, kStatSparseEvals, st_evals);
-    stat_add(A.stats, kStatSparseInserts, st_ins);
+    stat_add(A.stats, kStatSparseMerges, st_merges);
   }
   const size_t row = static_cast<size_t>(me) * k;
   if (done_reach == 0 ||
-      !heap_emit<KC>(D, J, k, min_rej, d.out_nn, d.out_d2, row)) {
+      !top_emit<KC>(h, k, min_rej, d.out_nn, d.out_d2, row)) {
     defer_source(s, A.defer_list, A.defer_count);
     return;
   }
@@ -1419,14 +1531,14 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
     k_zero_u32<<<1, 64, 0, st>>>(as<unsigned>(stats_.p), 2 * kStatCount);
     wa.stats = as<unsigned long long>(stats_.p);
   }
-  const unsigned dense_blocks =
-      static_cast<unsigned>((max_items + kDenseWaves - 1) / kDenseWaves);
   const unsigned sparse_blocks = blocks_for(ntot, 256);
-#define LC_WALK(KC)                                                         \
-  if (any_fast[KC]) {                                                       \
-    k_walk_dense<KC><<<dense_blocks, kDenseWaves * kWave, 0, st>>>(wa);     \
-    k_walk_sparse<KC><<<sparse_blocks, 256, 0, st>>>(wa);                   \
-    check(hipGetLastError(), "walk launch");                                \
+#define LC_WALK(KC)                                                           \
+  if (any_fast[KC]) {                                                         \
+    constexpr int W = WalkShape<KC>::kWaves;                                  \
+    k_walk_dense<KC><<<static_cast<unsigned>((max_items + W - 1) / W),        \
+                       W * kWave, 0, st>>>(wa);                               \
+    k_walk_sparse<KC><<<blocks_for(ntot, W * kWave), W * kWave, 0, st>>>(wa); \
+    check(hipGetLastError(), "walk launch");                                  \
   }
   LC_WALK(4)
   LC_WALK(8)
