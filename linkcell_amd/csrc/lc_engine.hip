@@ -62,6 +62,19 @@ constexpr int kTileShift = 4;
 constexpr int kChunkShift = 6;  // sources per dense work item (one wave)
 constexpr int kPad = 16;        // slack elements after SoA arrays (group reads)
 
+// Cell-ordered candidate record: folded coordinates and the point's index
+// within its frame (64-bit, so a record is 32 bytes: one s_load_dwordx8
+// when the address is wave-uniform, two dwordx4 loads otherwise).
+struct __align__(32) Cand {
+  double x, y, z;
+  long long id;
+};
+
+// Exact bounding box of a 16-point tile or of a cell, padded to 64 bytes.
+struct __align__(64) Box {
+  double lx, hx, ly, hy, lz, hz, pad0, pad1;
+};
+
 // r = 1 stencil in near-first order: home, 6 faces, 12 edges, 8 corners.
 // The canonical heap makes the result independent of the visit order
 // inside a shell; the order only makes the k-th distance tight early.
@@ -328,8 +341,7 @@ __global__ __launch_bounds__(256) void k_cell_sort(
     const int* __restrict__ items, const int* __restrict__ item_off,
     long long ncells, const int* __restrict__ off,
     const long long* __restrict__ tmp_key, const FrameDesc* __restrict__ fr,
-    const long long* __restrict__ fcb, int F, double* __restrict__ sx,
-    double* __restrict__ sy, double* __restrict__ sz, int* __restrict__ sidx,
+    const long long* __restrict__ fcb, int F, Cand* __restrict__ cand,
     int* __restrict__ scell, int* __restrict__ byidx) {
   const int w = uni(static_cast<int>(blockIdx.x * (blockDim.x / kWave) +
                                      threadIdx.x / kWave));
@@ -358,17 +370,18 @@ __global__ __launch_bounds__(256) void k_cell_sort(
   double s0, s1, s2, fx, fy, fz;
   fold_point(gm, r[0], r[1], r[2], s0, s1, s2, fx, fy, fz);
   const int dst = o0 + rk;
-  sx[dst] = fx;
-  sy[dst] = fy;
-  sz[dst] = fz;
-  sidx[dst] = mid;
+  Cand cr;
+  cr.x = fx;
+  cr.y = fy;
+  cr.z = fz;
+  cr.id = mid;
+  cand[dst] = cr;
   scell[dst] = g;
   byidx[o0 + ri] = dst;
 }
 
 // Exact bounding boxes. One wave per 64-member chunk = four 16-point
-// tiles; 16-lane groups reduce min / max with shuffles. Boxes are stored
-// SoA: [lo_x, hi_x, lo_y, hi_y, lo_z, hi_z] x cap.
+// tiles; 16-lane groups reduce min / max with shuffles.
 __device__ __forceinline__ double red16_min(double v) {
   for (int d = 8; d >= 1; d >>= 1) v = fmin(v, __shfl_xor(v, d, kWave));
   return v;
@@ -381,9 +394,8 @@ __device__ __forceinline__ double red16_max(double v) {
 __global__ __launch_bounds__(256) void k_tile_box(
     const int* __restrict__ items, const int* __restrict__ item_off,
     long long ncells, const int* __restrict__ off,
-    const int* __restrict__ tile_off, const double* __restrict__ sx,
-    const double* __restrict__ sy, const double* __restrict__ sz,
-    double* __restrict__ tbox, long long tcap) {
+    const int* __restrict__ tile_off, const Cand* __restrict__ cand,
+    Box* __restrict__ tbox) {
   const int w = uni(static_cast<int>(blockIdx.x * (blockDim.x / kWave) +
                                      threadIdx.x / kWave));
   const int lane = threadIdx.x & (kWave - 1);
@@ -396,40 +408,45 @@ __global__ __launch_bounds__(256) void k_tile_box(
   const int a = o0 + q * kWave + lane;
   const bool valid = a < o1;
   const double inf = as_f64(kInfBits);
-  const double x = valid ? sx[a] : 0.0, y = valid ? sy[a] : 0.0,
-               z = valid ? sz[a] : 0.0;
+  const Cand cr = cand[valid ? a : o0];
+  const double x = cr.x, y = cr.y, z = cr.z;
   const double lx = red16_min(valid ? x : inf), hx = red16_max(valid ? x : -inf);
   const double ly = red16_min(valid ? y : inf), hy = red16_max(valid ? y : -inf);
   const double lz = red16_min(valid ? z : inf), hz = red16_max(valid ? z : -inf);
   if ((lane & (kTile - 1)) == 0 && valid) {
     const long long t = tile_off[g] + (q * kWave + lane) / kTile;
-    tbox[0 * tcap + t] = lx;
-    tbox[1 * tcap + t] = hx;
-    tbox[2 * tcap + t] = ly;
-    tbox[3 * tcap + t] = hy;
-    tbox[4 * tcap + t] = lz;
-    tbox[5 * tcap + t] = hz;
+    Box b;
+    b.lx = lx;
+    b.hx = hx;
+    b.ly = ly;
+    b.hy = hy;
+    b.lz = lz;
+    b.hz = hz;
+    b.pad0 = 0.0;
+    b.pad1 = 0.0;
+    tbox[t] = b;
   }
 }
 
 __global__ __launch_bounds__(256) void k_cell_box(
     const int* __restrict__ off, const int* __restrict__ tile_off,
-    long long ncells, const double* __restrict__ tbox, long long tcap,
-    double* __restrict__ cbox) {
+    long long ncells, const Box* __restrict__ tbox, Box* __restrict__ cbox) {
   const long long g = blockIdx.x * static_cast<long long>(blockDim.x) +
                       threadIdx.x;
   if (g >= ncells) return;
   const int t0 = tile_off[g], t1 = tile_off[g + 1];
   if (t1 - t0 < 2) return;  // single-tile cells use the tile box
-  double b[6];
-  for (int c = 0; c < 6; ++c) b[c] = tbox[c * tcap + t0];
+  Box b = tbox[t0];
   for (int t = t0 + 1; t < t1; ++t) {
-    for (int c = 0; c < 6; c += 2) {
-      b[c] = fmin(b[c], tbox[c * tcap + t]);
-      b[c + 1] = fmax(b[c + 1], tbox[(c + 1) * tcap + t]);
-    }
+    const Box o = tbox[t];
+    b.lx = fmin(b.lx, o.lx);
+    b.hx = fmax(b.hx, o.hx);
+    b.ly = fmin(b.ly, o.ly);
+    b.hy = fmax(b.hy, o.hy);
+    b.lz = fmin(b.lz, o.lz);
+    b.hz = fmax(b.hz, o.hz);
   }
-  for (int c = 0; c < 6; ++c) cbox[c * ncells + g] = b[c];
+  cbox[g] = b;
 }
 
 // ------------------------------------------------------------ heap
@@ -574,10 +591,9 @@ __device__ __forceinline__ void defer_source(int s, int* defer_list,
 // corner differences and the bound never exceeds the d2 the walk would
 // compute: pruning on it cannot change any result.
 __device__ __forceinline__ double box_lb(double px, double py, double pz,
-                                         double lx, double hx, double ly,
-                                         double hy, double lz, double hz,
-                                         double shx, double shy, double shz,
-                                         bool zero) {
+                                         const Box& b, double shx, double shy,
+                                         double shz, bool zero) {
+  double lx = b.lx, hx = b.hx, ly = b.ly, hy = b.hy, lz = b.lz, hz = b.hz;
   if (!zero) {
     lx = lx + shx;
     hx = hx + shx;
@@ -597,19 +613,15 @@ struct WalkArgs {
   const long long* fcb;
   int F;
   const int* fflags;
-  const double* sx;
-  const double* sy;
-  const double* sz;
-  const int* sidx;
+  const Cand* cand;
   const int* scell;
   const int* off;
   const int* tile_off;
   const int* items;
   const int* item_off;
   long long ncells;
-  const double* tbox;
-  long long tcap;
-  const double* cbox;
+  const Box* tbox;
+  const Box* cbox;
   int dense_min;
   int* defer_list;
   int* defer_count;
@@ -660,7 +672,7 @@ __device__ __forceinline__ bool shell_offset(int r, int idx, int& dx, int& dy,
 template <int KC>
 struct WalkShape {
   static constexpr int kWaves = KC >= 32 ? 2 : 4;
-  static constexpr int kGroup = KC >= 16 ? 8 : KC / 2;  // appends per check
+  static constexpr int kGroup = KC >= 8 ? 4 : 2;  // candidates per buffer check
 };
 
 // ---------------------------------------------------------- dense walk
@@ -706,8 +718,9 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_dense(
   const int hx = c % nx;
   const int hy = (c / nx) % ny;
   const int hz = c / (nx * ny);
-  const double px = A.sx[s], py = A.sy[s], pz = A.sz[s];
-  const int me = A.sidx[s];
+  const Cand mc = A.cand[s];
+  const double px = mc.x, py = mc.y, pz = mc.z;
+  const int me = static_cast<int>(mc.id);
   const int ortho = d.ortho;
   const double w0 = d.widths[0], w1 = d.widths[1], w2 = d.widths[2];
 
@@ -759,11 +772,7 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_dense(
     }
     const int nt = (n2 + kTile - 1) >> kTileShift;
     if (nt > 1) {
-      const long long nc6 = A.ncells;
-      const double lb = box_lb(px, py, pz, A.cbox[0 * nc6 + gq],
-                               A.cbox[1 * nc6 + gq], A.cbox[2 * nc6 + gq],
-                               A.cbox[3 * nc6 + gq], A.cbox[4 * nc6 + gq],
-                               A.cbox[5 * nc6 + gq], shx, shy, shz, zero);
+      const double lb = box_lb(px, py, pz, A.cbox[gq], shx, shy, shz, zero);
       if (!__ballot(live && lb <= thr)) return;
     }
     const int t0 = uni(A.tile_off[gq]);
@@ -773,10 +782,7 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_dense(
       int t = start + tt;
       t = t >= nt ? t - nt : t;
       const long long ti = t0 + t;
-      const double lb = box_lb(px, py, pz, A.tbox[0 * A.tcap + ti],
-                               A.tbox[1 * A.tcap + ti], A.tbox[2 * A.tcap + ti],
-                               A.tbox[3 * A.tcap + ti], A.tbox[4 * A.tcap + ti],
-                               A.tbox[5 * A.tcap + ti], shx, shy, shz, zero);
+      const double lb = box_lb(px, py, pz, A.tbox[ti], shx, shy, shz, zero);
       const bool need = live && lb <= thr;
       st_tests += 1;
       if (!__ballot(need)) continue;
@@ -791,8 +797,9 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_dense(
         for (int u = 0; u < G; ++u) {
           if (u < lim) {
             const int pos = base + hh + u;
-            double qx = A.sx[pos], qy = A.sy[pos], qz = A.sz[pos];
-            const int j = A.sidx[pos];
+            const Cand cq = A.cand[pos];
+            double qx = cq.x, qy = cq.y, qz = cq.z;
+            const int j = static_cast<int>(cq.id);
             if (!zero) {
               qx = qx + shx;
               qy = qy + shy;
@@ -896,8 +903,9 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
   const int hx = c % nx;
   const int hy = (c / nx) % ny;
   const int hz = c / (nx * ny);
-  const double px = A.sx[s], py = A.sy[s], pz = A.sz[s];
-  const int me = A.sidx[s];
+  const Cand mc = A.cand[s];
+  const double px = mc.x, py = mc.y, pz = mc.z;
+  const int me = static_cast<int>(mc.id);
   const int own_tile = (s - o0) >> kTileShift;
   const int ortho = d.ortho;
   const double w0 = d.widths[0], w1 = d.widths[1], w2 = d.widths[2];
@@ -952,11 +960,7 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
     }
     const int nt = (n2 + kTile - 1) >> kTileShift;
     if (nt > 1) {
-      const long long nc6 = A.ncells;
-      const double lb = box_lb(px, py, pz, A.cbox[0 * nc6 + gq],
-                               A.cbox[1 * nc6 + gq], A.cbox[2 * nc6 + gq],
-                               A.cbox[3 * nc6 + gq], A.cbox[4 * nc6 + gq],
-                               A.cbox[5 * nc6 + gq], shx, shy, shz, zero);
+      const double lb = box_lb(px, py, pz, A.cbox[gq], shx, shy, shz, zero);
       if (lb > thr) return;
     }
     const int t0 = A.tile_off[gq];
@@ -967,12 +971,7 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
       t = t >= nt ? t - nt : t;
       const long long ti = t0 + t;
       if (nt > 1 || !home) {
-        const double lb = box_lb(px, py, pz, A.tbox[0 * A.tcap + ti],
-                                 A.tbox[1 * A.tcap + ti],
-                                 A.tbox[2 * A.tcap + ti],
-                                 A.tbox[3 * A.tcap + ti],
-                                 A.tbox[4 * A.tcap + ti],
-                                 A.tbox[5 * A.tcap + ti], shx, shy, shz, zero);
+        const double lb = box_lb(px, py, pz, A.tbox[ti], shx, shy, shz, zero);
         st_tests += 1;
         if (lb > thr) continue;
       }
@@ -987,10 +986,11 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
 #pragma unroll
         for (int u = 0; u < G; ++u) {
           const int pos = base + hh + (u < lim ? u : 0);
-          qx[u] = A.sx[pos];
-          qy[u] = A.sy[pos];
-          qz[u] = A.sz[pos];
-          jj[u] = A.sidx[pos];
+          const Cand cq = A.cand[pos];
+          qx[u] = cq.x;
+          qy[u] = cq.y;
+          qz[u] = cq.z;
+          jj[u] = static_cast<int>(cq.id);
         }
 #pragma unroll
         for (int u = 0; u < G; ++u) {
@@ -1079,10 +1079,8 @@ __device__ __forceinline__ long long total_key(double x) {
 // One lane per source; a grid-stride loop over the deferred list.
 __global__ __launch_bounds__(64) void k_walk_exact(
     const FrameDesc* __restrict__ fr, const long long* __restrict__ fcb,
-    int F, const double* __restrict__ sx, const double* __restrict__ sy,
-    const double* __restrict__ sz, const int* __restrict__ sidx,
-    const int* __restrict__ scell, const int* __restrict__ off,
-    const int* __restrict__ byidx, const int* __restrict__ defer_list,
+    int F, const Cand* __restrict__ cand, const int* __restrict__ scell,
+    const int* __restrict__ off, const int* __restrict__ byidx, const int* __restrict__ defer_list,
     const int* __restrict__ defer_count, double* __restrict__ hd2,
     int* __restrict__ hidx) {
   const long long tid = blockIdx.x * static_cast<long long>(blockDim.x) +
@@ -1100,8 +1098,9 @@ __global__ __launch_bounds__(64) void k_walk_exact(
     const long long cb = d.cell_base;
     const int c = static_cast<int>(g - cb);
     const int hx = c % nx, hy = (c / nx) % ny, hz = c / (nx * ny);
-    const double px = sx[s], py = sy[s], pz = sz[s];
-    const int me = sidx[s];
+    const Cand mc = cand[s];
+    const double px = mc.x, py = mc.y, pz = mc.z;
+    const int me = static_cast<int>(mc.id);
     double* hd = hd2 + tid;
     int* hj = hidx + tid;
     const long long st = nthreads;
@@ -1125,11 +1124,11 @@ __global__ __launch_bounds__(64) void k_walk_exact(
             const int a0 = off[cb + cq];
             const int a1 = off[cb + cq + 1];
             for (int t = a1 - 1; t >= a0; --t) {
-              const int pos = byidx[t];
-              const int j = sidx[pos];
+              const Cand cq = cand[byidx[t]];
+              const int j = static_cast<int>(cq.id);
               if (j == me) continue;
-              const double d2 = dist2_shifted(px, py, pz, sx[pos], sy[pos],
-                                              sz[pos], shx, shy, shz);
+              const double d2 = dist2_shifted(px, py, pz, cq.x, cq.y, cq.z,
+                                              shx, shy, shz);
               // KHeap::push. With a full heap, d2 >= the worst slot can
               // change nothing: an index already present holds at most
               // that value, and the replacement needs d2 < worst.
@@ -1286,8 +1285,8 @@ void Engine::grow(Buf& b, size_t bytes) {
 void Engine::free_all() {
   Buf* all[] = {&frames_,   &fpt_,      &fcb_,    &count_,  &status_,
                 &counters_, &off_,      &tile_off_, &item_off_, &items_,
-                &cellslot_, &tmp_key_,  &sx_,     &sy_,     &sz_,
-                &sidx_,     &scell_,    &byidx_,  &tbox_,   &cbox_,
+                &cellslot_, &tmp_key_,  &cand_,
+                &scell_,    &byidx_,  &tbox_,   &cbox_,
                 &defer_,    &heap_d2_,  &heap_idx_, &flags_, &stats_};
   for (Buf* b : all) {
     if (b->p != nullptr) (void)hipFree(b->p);
@@ -1385,14 +1384,11 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   grow(items_, sizeof(int) * static_cast<size_t>(max_items));
   grow(cellslot_, sizeof(int4) * static_cast<size_t>(ntot));
   grow(tmp_key_, sizeof(long long) * static_cast<size_t>(ntot));
-  grow(sx_, sizeof(double) * static_cast<size_t>(ntot + kPad));
-  grow(sy_, sizeof(double) * static_cast<size_t>(ntot + kPad));
-  grow(sz_, sizeof(double) * static_cast<size_t>(ntot + kPad));
-  grow(sidx_, sizeof(int) * static_cast<size_t>(ntot + kPad));
+  grow(cand_, sizeof(Cand) * static_cast<size_t>(ntot + kPad));
   grow(scell_, sizeof(int) * static_cast<size_t>(ntot));
   grow(byidx_, sizeof(int) * static_cast<size_t>(ntot));
-  grow(tbox_, sizeof(double) * 6 * static_cast<size_t>(tcap));
-  grow(cbox_, sizeof(double) * 6 * static_cast<size_t>(ctot));
+  grow(tbox_, sizeof(Box) * static_cast<size_t>(tcap));
+  grow(cbox_, sizeof(Box) * static_cast<size_t>(ctot));
   grow(defer_, sizeof(int) * static_cast<size_t>(ntot));
   grow(flags_, sizeof(int) * static_cast<size_t>(F));
   // exact-walk scratch: one heap of kmax slots per resident lane
@@ -1491,16 +1487,14 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   const unsigned item_blocks = blocks_for(max_items * kWave, 256);
   k_cell_sort<<<item_blocks, 256, 0, st>>>(
       as<int>(items_.p), as<int>(item_off_.p), ctot, as<int>(off_.p),
-      as<long long>(tmp_key_.p), dfr, dfcb, F, as<double>(sx_.p),
-      as<double>(sy_.p), as<double>(sz_.p), as<int>(sidx_.p),
+      as<long long>(tmp_key_.p), dfr, dfcb, F, as<Cand>(cand_.p),
       as<int>(scell_.p), as<int>(byidx_.p));
   k_tile_box<<<item_blocks, 256, 0, st>>>(
       as<int>(items_.p), as<int>(item_off_.p), ctot, as<int>(off_.p),
-      as<int>(tile_off_.p), as<double>(sx_.p), as<double>(sy_.p),
-      as<double>(sz_.p), as<double>(tbox_.p), tcap);
+      as<int>(tile_off_.p), as<Cand>(cand_.p), as<Box>(tbox_.p));
   k_cell_box<<<blocks_for(ctot, 256), 256, 0, st>>>(
-      as<int>(off_.p), as<int>(tile_off_.p), ctot, as<double>(tbox_.p), tcap,
-      as<double>(cbox_.p));
+      as<int>(off_.p), as<int>(tile_off_.p), ctot, as<Box>(tbox_.p),
+      as<Box>(cbox_.p));
   check(hipGetLastError(), "cell_sort launch");
   if (timing_) check(hipEventRecord(ev_[4], st), "event record");
 
@@ -1509,19 +1503,15 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   wa.fcb = dfcb;
   wa.F = F;
   wa.fflags = as<int>(flags_.p);
-  wa.sx = as<double>(sx_.p);
-  wa.sy = as<double>(sy_.p);
-  wa.sz = as<double>(sz_.p);
-  wa.sidx = as<int>(sidx_.p);
+  wa.cand = as<Cand>(cand_.p);
   wa.scell = as<int>(scell_.p);
   wa.off = as<int>(off_.p);
   wa.tile_off = as<int>(tile_off_.p);
   wa.items = as<int>(items_.p);
   wa.item_off = as<int>(item_off_.p);
   wa.ncells = ctot;
-  wa.tbox = as<double>(tbox_.p);
-  wa.tcap = tcap;
-  wa.cbox = as<double>(cbox_.p);
+  wa.tbox = as<Box>(tbox_.p);
+  wa.cbox = as<Box>(cbox_.p);
   wa.dense_min = dense_min_;
   wa.defer_list = as<int>(defer_.p);
   wa.defer_count = counters + 1;
@@ -1558,8 +1548,8 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   }
   if (timing_) check(hipEventRecord(ev_[5], st), "event record");
   k_walk_exact<<<static_cast<unsigned>(exact_threads / 64), 64, 0, st>>>(
-      dfr, dfcb, F, as<double>(sx_.p), as<double>(sy_.p), as<double>(sz_.p),
-      as<int>(sidx_.p), as<int>(scell_.p), as<int>(off_.p), as<int>(byidx_.p),
+      dfr, dfcb, F, as<Cand>(cand_.p), as<int>(scell_.p), as<int>(off_.p),
+      as<int>(byidx_.p),
       as<int>(defer_.p), counters + 1, as<double>(heap_d2_.p),
       as<int>(heap_idx_.p));
   check(hipGetLastError(), "exact walk launch");

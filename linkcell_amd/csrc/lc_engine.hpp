@@ -100,7 +100,7 @@ class Engine {
   hipStream_t stream_ = nullptr;
   int device_ = -1;
   Buf frames_, fpt_, fcb_, count_, status_, counters_, off_, tile_off_,
-      item_off_, items_, cellslot_, tmp_key_, sx_, sy_, sz_, sidx_, scell_,
+      item_off_, items_, cellslot_, tmp_key_, cand_, scell_,
       byidx_, tbox_, cbox_, defer_, heap_d2_, heap_idx_, flags_, stats_;
   void* staging_ = nullptr;
   size_t staging_bytes_ = 0;
