@@ -39,6 +39,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #pragma clang fp contract(off)
 
@@ -743,6 +744,64 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_dense(
     thr = as_f64(h.d[KC - 1]);
   };
 
+  // Tiles of one candidate cell, starting at tile `start`. The next
+  // tile's box and the next group of candidates are loaded before the
+  // current ones are used, so the scalar loads overlap the arithmetic.
+  // Z: the cell needs no lattice shift (q + 0 == q for the d2 bits).
+  auto walk_tiles = [&](auto ztag, int a0, int n2, int t0, int nt, int start,
+                        bool home, double shx, double shy, double shz) {
+    constexpr bool Z = decltype(ztag)::value;
+    int t = start;
+    Box bx = A.tbox[t0 + t];
+#pragma unroll 1
+    for (int tt = 0; tt < nt; ++tt) {
+      int tn = t + 1;
+      tn = tn >= nt ? tn - nt : tn;
+      const Box bnext = A.tbox[t0 + tn];
+      const double lb = box_lb(px, py, pz, bx, shx, shy, shz, Z);
+      const bool need = live && lb <= thr;
+      st_tests += 1;
+      if (__ballot(need)) {
+        st_tiles += 1;
+        const int base = a0 + t * kTile;
+        const int e = min(kTile, n2 - t * kTile);
+        const int last = base + e - 1;
+        Cand c[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) c[u] = A.cand[min(base + u, last)];
+#pragma unroll 1
+        for (int hh = 0; hh < e; hh += G) {
+          Cand nxt[G];
+#pragma unroll
+          for (int u = 0; u < G; ++u) nxt[u] = A.cand[min(base + hh + G + u, last)];
+          const int lim = e - hh;
+          st_steps += lim < G ? lim : G;
+#pragma unroll
+          for (int u = 0; u < G; ++u) {
+            const double qx = Z ? c[u].x : c[u].x + shx;
+            const double qy = Z ? c[u].y : c[u].y + shy;
+            const double qz = Z ? c[u].z : c[u].z + shz;
+            const int j = static_cast<int>(c[u].id);
+            const double ddx = qx - px, ddy = qy - py, ddz = qz - pz;
+            const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            const bool acc = u < lim && need && d2 <= thr && !(home && j == me);
+            if (acc) {
+              bd[nbuf * kWave] = __double_as_longlong(d2);
+              bj[nbuf * kWave] = j;
+              nbuf += 1;
+              st_appends += 1;
+            }
+          }
+          if (__ballot(nbuf > KC - G)) flush();
+#pragma unroll
+          for (int u = 0; u < G; ++u) c[u] = nxt[u];
+        }
+      }
+      t = tn;
+      bx = bnext;
+    }
+  };
+
   // Visit one candidate cell (unwrapped offset dx, dy, dz from home).
   auto visit = [&](int dx, int dy, int dz, bool home) {
     const int jx = hx + dx, jy = hy + dy, jz = hz + dz;
@@ -776,48 +835,15 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_dense(
       if (!__ballot(live && lb <= thr)) return;
     }
     const int t0 = uni(A.tile_off[gq]);
-    const int start = home ? (q * (kWave / kTile)) % nt : 0;
-#pragma unroll 1
-    for (int tt = 0; tt < nt; ++tt) {
-      int t = start + tt;
-      t = t >= nt ? t - nt : t;
-      const long long ti = t0 + t;
-      const double lb = box_lb(px, py, pz, A.tbox[ti], shx, shy, shz, zero);
-      const bool need = live && lb <= thr;
-      st_tests += 1;
-      if (!__ballot(need)) continue;
-      st_tiles += 1;
-      const int base = a0 + t * kTile;
-      const int e = min(kTile, n2 - t * kTile);
-#pragma unroll 1
-      for (int hh = 0; hh < e; hh += G) {
-        const int lim = e - hh;
-        st_steps += lim < G ? lim : G;
-#pragma unroll
-        for (int u = 0; u < G; ++u) {
-          if (u < lim) {
-            const int pos = base + hh + u;
-            const Cand cq = A.cand[pos];
-            double qx = cq.x, qy = cq.y, qz = cq.z;
-            const int j = static_cast<int>(cq.id);
-            if (!zero) {
-              qx = qx + shx;
-              qy = qy + shy;
-              qz = qz + shz;
-            }
-            const double ddx = qx - px, ddy = qy - py, ddz = qz - pz;
-            const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            const bool acc = need && d2 <= thr && !(home && j == me);
-            if (acc) {
-              bd[nbuf * kWave] = __double_as_longlong(d2);
-              bj[nbuf * kWave] = j;
-              nbuf += 1;
-              st_appends += 1;
-            }
-          }
-        }
-        if (__ballot(nbuf > KC - G)) flush();
-      }
+    int start = 0;
+    if (home) {
+      start = q * (kWave / kTile);
+      while (start >= nt) start -= nt;
+    }
+    if (zero) {
+      walk_tiles(std::true_type{}, a0, n2, t0, nt, start, home, shx, shy, shz);
+    } else {
+      walk_tiles(std::false_type{}, a0, n2, t0, nt, start, home, shx, shy, shz);
     }
   };
 
@@ -932,6 +958,56 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
     }
   };
 
+  // One lane's tiles of one candidate cell from tile `start`; the next
+  // group of candidates is loaded before the current group is used.
+  auto walk_tiles = [&](auto ztag, int a0, int n2, int t0, int nt, int start,
+                        bool home, double shx, double shy, double shz) {
+    constexpr bool Z = decltype(ztag)::value;
+#pragma unroll 1
+    for (int tt = 0; tt < nt; ++tt) {
+      int t = start + tt;
+      t = t >= nt ? t - nt : t;
+      if (nt > 1 || !home) {
+        const double lb = box_lb(px, py, pz, A.tbox[t0 + t], shx, shy, shz, Z);
+        st_tests += 1;
+        if (lb > thr) continue;
+      }
+      st_tiles += 1;
+      const int base = a0 + t * kTile;
+      const int e = min(kTile, n2 - t * kTile);
+      const int last = base + e - 1;
+      Cand c[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) c[u] = A.cand[min(base + u, last)];
+#pragma unroll 1
+      for (int hh = 0; hh < e; hh += G) {
+        Cand nxt[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) nxt[u] = A.cand[min(base + hh + G + u, last)];
+        const int lim = e - hh;
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+          const double qx = Z ? c[u].x : c[u].x + shx;
+          const double qy = Z ? c[u].y : c[u].y + shy;
+          const double qz = Z ? c[u].z : c[u].z + shz;
+          const int j = static_cast<int>(c[u].id);
+          const double ddx = qx - px, ddy = qy - py, ddz = qz - pz;
+          const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+          const bool acc = u < lim && d2 <= thr && j != me;
+          if (acc) {
+            bd[nbuf * kWave] = __double_as_longlong(d2);
+            bj[nbuf * kWave] = j;
+            nbuf += 1;
+          }
+        }
+        st_evals += lim < G ? lim : G;
+        if (__ballot(nbuf > KC - G)) flush();
+#pragma unroll
+        for (int u = 0; u < G; ++u) c[u] = nxt[u];
+      }
+    }
+  };
+
   auto visit = [&](int dx, int dy, int dz, bool home) {
     const int jx = hx + dx, jy = hy + dy, jz = hz + dz;
     const int cx = jx < 0 ? jx + nx : (jx >= nx ? jx - nx : jx);
@@ -965,53 +1041,10 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
     }
     const int t0 = A.tile_off[gq];
     const int start = home ? own_tile : 0;
-#pragma unroll 1
-    for (int tt = 0; tt < nt; ++tt) {
-      int t = start + tt;
-      t = t >= nt ? t - nt : t;
-      const long long ti = t0 + t;
-      if (nt > 1 || !home) {
-        const double lb = box_lb(px, py, pz, A.tbox[ti], shx, shy, shz, zero);
-        st_tests += 1;
-        if (lb > thr) continue;
-      }
-      st_tiles += 1;
-      const int base = a0 + t * kTile;
-      const int e = min(kTile, n2 - t * kTile);
-#pragma unroll 1
-      for (int hh = 0; hh < e; hh += G) {
-        const int lim = e - hh;
-        double qx[G], qy[G], qz[G];
-        int jj[G];
-#pragma unroll
-        for (int u = 0; u < G; ++u) {
-          const int pos = base + hh + (u < lim ? u : 0);
-          const Cand cq = A.cand[pos];
-          qx[u] = cq.x;
-          qy[u] = cq.y;
-          qz[u] = cq.z;
-          jj[u] = static_cast<int>(cq.id);
-        }
-#pragma unroll
-        for (int u = 0; u < G; ++u) {
-          double ax = qx[u], ay = qy[u], az = qz[u];
-          if (!zero) {
-            ax = ax + shx;
-            ay = ay + shy;
-            az = az + shz;
-          }
-          const double ddx = ax - px, ddy = ay - py, ddz = az - pz;
-          const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-          const bool acc = u < lim && d2 <= thr && jj[u] != me;
-          if (acc) {
-            bd[nbuf * kWave] = __double_as_longlong(d2);
-            bj[nbuf * kWave] = jj[u];
-            nbuf += 1;
-          }
-        }
-        st_evals += lim < G ? lim : G;
-        if (__ballot(nbuf > KC - G)) flush();
-      }
+    if (zero) {
+      walk_tiles(std::true_type{}, a0, n2, t0, nt, start, home, shx, shy, shz);
+    } else {
+      walk_tiles(std::false_type{}, a0, n2, t0, nt, start, home, shx, shy, shz);
     }
   };
 
