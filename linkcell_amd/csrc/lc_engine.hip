@@ -913,6 +913,7 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
   const int wv = threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   const int nact = A.off[A.ncells];
+  const unsigned long long t_begin = wall_clock64();
   const int s0 = blockIdx.x * blockDim.x + threadIdx.x;
   const int s = s0 < nact ? s0 : 0;
   // lane states
@@ -1086,6 +1087,12 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
         }
       }
     }
+  }
+  if (A.stats != nullptr && lane == 0) {
+    // per-wave timeline (100 MHz wall clock) after the counters
+    const long long wid = blockIdx.x * static_cast<long long>(W) + wv;
+    A.stats[kStatCount + 2 * wid] = t_begin;
+    A.stats[kStatCount + 2 * wid + 1] = wall_clock64();
   }
   if (A.stats != nullptr && s0 < nact) {
     stat_add(A.stats, kStatSparseSources, 1);
@@ -1272,13 +1279,13 @@ unsigned blocks_for(long long n, int threads) {
 
 void Engine::read_stats(unsigned long long* out, int n) {
   sync();
-  std::vector<unsigned long long> h(kStatCount, 0);
-  if (stats_.p != nullptr) {
-    check(hipMemcpy(h.data(), stats_.p, sizeof(unsigned long long) * kStatCount,
+  const long long m = std::min<long long>(n, stats_n_);
+  if (stats_.p != nullptr && m > 0) {
+    check(hipMemcpy(out, stats_.p, sizeof(unsigned long long) * static_cast<size_t>(m),
                     hipMemcpyDeviceToHost),
           "DtoH stats");
   }
-  for (int i = 0; i < n && i < kStatCount; ++i) out[i] = h[static_cast<size_t>(i)];
+  for (long long i = std::max<long long>(m, 0); i < n; ++i) out[i] = 0;
 }
 
 int fast_bucket(int k) {
@@ -1573,8 +1580,11 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   wa.defer_count = counters + 1;
   wa.stats = nullptr;
   if (stats_on_) {
-    grow(stats_, sizeof(unsigned long long) * kStatCount);
-    k_zero_u32<<<1, 64, 0, st>>>(as<unsigned>(stats_.p), 2 * kStatCount);
+    const long long nstat = kStatCount + 2 * (ntot / kWave + 1);
+    grow(stats_, sizeof(unsigned long long) * static_cast<size_t>(nstat));
+    k_zero_u32<<<blocks_for(2 * nstat, 256), 256, 0, st>>>(
+        as<unsigned>(stats_.p), 2 * nstat);
+    stats_n_ = nstat;
     wa.stats = as<unsigned long long>(stats_.p);
   }
   const unsigned sparse_blocks = blocks_for(ntot, 256);

@@ -111,6 +111,7 @@ class Engine {
   WalkMode mode_ = WalkMode::kAuto;
   int dense_min_ = 32;
   bool stats_on_ = false;
+  long long stats_n_ = 0;
   hipEvent_t ev_[8] = {};
   Timings timings_{};
 };
