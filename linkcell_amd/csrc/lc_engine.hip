@@ -893,47 +893,46 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_dense(
 // --------------------------------------------------------- sparse walk
 
 // One lane per source of a cell below the dense threshold, in cell order
-// (neighbouring lanes share home or adjacent cells). Each lane is a small
-// state machine over its own stencil: the home cell from its own 16-point
-// tile outwards, then the near-first neighbour order; cells and tiles
-// whose exact lower bound exceeds the lane's k-th best are skipped. One
-// loop iteration advances every lane by one step (a cell header, one tile
-// test, or one group of G candidates), so lanes with pruned tiles do not
-// idle while others walk theirs. Accepted candidates go to the lane's LDS
-// buffer; merges and stop tests happen for all lanes together, when a
-// buffer is nearly full, when a quarter of the lanes have finished their
-// shell, or when no lane has work left.
+// (neighbouring lanes share home or adjacent cells). Each lane walks its
+// own stencil: the home cell from its own 16-point tile outwards, then
+// the near-first neighbour order; cells and tiles whose exact lower bound
+// exceeds the lane's k-th best are skipped. Accepted candidates go to the
+// lane's LDS buffer; whenever any lane's buffer is nearly full, every lane
+// with entries merges its batch.
 template <int KC>
 __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
     WalkArgs A) {
   constexpr int W = WalkShape<KC>::kWaves;
-  constexpr int G = 4;
+  constexpr int G = WalkShape<KC>::kGroup;
   __shared__ long long s_bd[W][KC][kWave];
   __shared__ int s_bj[W][KC][kWave];
   const int wv = threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
-  const int nact = A.off[A.ncells];
   const unsigned long long t_begin = wall_clock64();
-  const int s0 = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = s0 < nact ? s0 : 0;
-  // lane states
-  constexpr int kWalk = 0, kWaitStop = 1, kDone = 2;
-  int state = s0 < nact ? kWalk : kDone;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nact = A.off[A.ncells];
+  if (A.stats != nullptr && lane == 0) {
+    // per-wave timeline (100 MHz wall clock) after the counters; the
+    // end stamp is written by the wave's last lane to finish
+    const long long wid = blockIdx.x * static_cast<long long>(W) + wv;
+    A.stats[kStatCount + 2 * wid] = t_begin;
+  }
+  if (s >= nact) return;
   const int g = A.scell[s];
   const int o0 = A.off[g];
   const int cnt = A.off[g + 1] - o0;
-  if (cnt >= A.dense_min) state = kDone;
+  if (cnt >= A.dense_min) return;
   const int f = find_frame(A.fcb, A.F, g);
   const FrameDesc& d = A.fr[f];
-  if (d.kc != KC) state = kDone;
-  if (state == kWalk && ((A.fflags[f] & kFrameNonFinite) != 0 || d.r_safe < 1)) {
+  if (d.kc != KC) return;
+  if ((A.fflags[f] & kFrameNonFinite) != 0 || d.r_safe < 1) {
     defer_source(s, A.defer_list, A.defer_count);
-    state = kDone;
+    return;
   }
   const int k = d.k;
   const int nx = d.nx, ny = d.ny, nz = d.nz;
-  const int cb = static_cast<int>(d.cell_base);
-  const int c = g - cb;
+  const long long cb = d.cell_base;
+  const int c = g - static_cast<int>(cb);
   const int hx = c % nx;
   const int hy = (c / nx) % ny;
   const int hz = c / (nx * ny);
@@ -941,8 +940,11 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
   const double px = mc.x, py = mc.y, pz = mc.z;
   const int me = static_cast<int>(mc.id);
   const int own_tile = (s - o0) >> kTileShift;
-  const int r_safe = d.r_safe;
-  const double cell_min = d.cell_min;
+  const int ortho = d.ortho;
+  const double w0 = d.widths[0], w1 = d.widths[1], w2 = d.widths[2];
+  double hm[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) hm[t] = d.h[t];
 
   Top<KC> h;
   top_init<KC>(h, k);
@@ -950,163 +952,140 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
   double thr = as_f64(kInfBits);
   int nbuf = 0;
   int done_reach = 0;
-  bool ok = false;
   long long* bd = &s_bd[wv][0][lane];
   int* bj = &s_bj[wv][0][lane];
   unsigned st_tests = 0, st_tiles = 0, st_evals = 0, st_merges = 0;
 
-  // walk cursor
-  int r = 1, ci = -1, ncube = 27;
-  int nt = 0, tt = 0, start = 0, t0 = 0, a0 = 0, n2 = 0;
-  bool home = false;
-  double shx = 0.0, shy = 0.0, shz = 0.0;
-  int cur = 0, end = 0;
+  auto flush = [&]() {
+    if (nbuf > 0) {
+      st_merges += 1;
+      batch_merge<KC>(h, bd, bj, nbuf, min_rej);
+      nbuf = 0;
+      thr = as_f64(h.d[KC - 1]);
+    }
+  };
+
+  // One lane's tiles of one candidate cell from tile `start`; the next
+  // group of candidates is loaded before the current group is used.
+  auto walk_tiles = [&](auto ztag, int a0, int n2, int t0, int nt, int start,
+                        bool home, double shx, double shy, double shz) {
+    constexpr bool Z = decltype(ztag)::value;
+#pragma unroll 1
+    for (int tt = 0; tt < nt; ++tt) {
+      int t = start + tt;
+      t = t >= nt ? t - nt : t;
+      if (nt > 1 || !home) {
+        const double lb = box_lb(px, py, pz, A.tbox[t0 + t], shx, shy, shz, Z);
+        st_tests += 1;
+        if (lb > thr) continue;
+      }
+      st_tiles += 1;
+      const int base = a0 + t * kTile;
+      const int e = min(kTile, n2 - t * kTile);
+      const int last = base + e - 1;
+      Cand c[G];
+#pragma unroll
+      for (int u = 0; u < G; ++u) c[u] = A.cand[min(base + u, last)];
+#pragma unroll 1
+      for (int hh = 0; hh < e; hh += G) {
+        Cand nxt[G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) nxt[u] = A.cand[min(base + hh + G + u, last)];
+        const int lim = e - hh;
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+          const double qx = Z ? c[u].x : c[u].x + shx;
+          const double qy = Z ? c[u].y : c[u].y + shy;
+          const double qz = Z ? c[u].z : c[u].z + shz;
+          const int j = static_cast<int>(c[u].id);
+          const double ddx = qx - px, ddy = qy - py, ddz = qz - pz;
+          const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+          const bool acc = u < lim && d2 <= thr && j != me;
+          if (acc) {
+            bd[nbuf * kWave] = __double_as_longlong(d2);
+            bj[nbuf * kWave] = j;
+            nbuf += 1;
+          }
+        }
+        st_evals += lim < G ? lim : G;
+        if (__ballot(nbuf > KC - G)) flush();
+#pragma unroll
+        for (int u = 0; u < G; ++u) c[u] = nxt[u];
+      }
+    }
+  };
+
+  auto visit = [&](int dx, int dy, int dz, bool home) {
+    const int jx = hx + dx, jy = hy + dy, jz = hz + dz;
+    const int cx = jx < 0 ? jx + nx : (jx >= nx ? jx - nx : jx);
+    const int cy = jy < 0 ? jy + ny : (jy >= ny ? jy - ny : jy);
+    const int cz = jz < 0 ? jz + nz : (jz >= nz ? jz - nz : jz);
+    const int na = jx < 0 ? -1 : (jx >= nx ? 1 : 0);
+    const int nb = jy < 0 ? -1 : (jy >= ny ? 1 : 0);
+    const int nc = jz < 0 ? -1 : (jz >= nz ? 1 : 0);
+    const int gq = static_cast<int>(cb) + (cz * ny + cy) * nx + cx;
+    const int a0 = A.off[gq];
+    const int n2 = A.off[gq + 1] - a0;
+    if (n2 == 0) return;
+    const bool zero = (na | nb | nc) == 0;
+    double shx = 0.0, shy = 0.0, shz = 0.0;
+    if (!zero) {
+      if (ortho) {
+        shx = static_cast<double>(na) * w0;
+        shy = static_cast<double>(nb) * w1;
+        shz = static_cast<double>(nc) * w2;
+      } else {
+        const double fa = na, fb = nb, fc = nc;
+        shx = fa * hm[0] + fb * hm[3] + fc * hm[6];
+        shy = fa * hm[1] + fb * hm[4] + fc * hm[7];
+        shz = fa * hm[2] + fb * hm[5] + fc * hm[8];
+      }
+    }
+    const int nt = (n2 + kTile - 1) >> kTileShift;
+    if (nt > 1) {
+      const double lb = box_lb(px, py, pz, A.cbox[gq], shx, shy, shz, zero);
+      if (lb > thr) return;
+    }
+    const int t0 = A.tile_off[gq];
+    const int start = home ? own_tile : 0;
+    if (zero) {
+      walk_tiles(std::true_type{}, a0, n2, t0, nt, start, home, shx, shy, shz);
+    } else {
+      walk_tiles(std::false_type{}, a0, n2, t0, nt, start, home, shx, shy, shz);
+    }
+  };
 
 #pragma unroll 1
-  while (__ballot(state != kDone)) {
-    // --- advance: a cell header or one tile test
-    if (state == kWalk && cur >= end) {
-      if (tt < nt) {
-        int t = start + tt;
-        t = t >= nt ? t - nt : t;
-        tt += 1;
-        bool need = true;
-        if (nt > 1 || !home) {
-          const double lb = box_lb(px, py, pz, A.tbox[t0 + t], shx, shy, shz, false);
-          st_tests += 1;
-          need = lb <= thr;
-        }
-        if (need) {
-          st_tiles += 1;
-          cur = a0 + t * kTile;
-          end = cur + min(kTile, n2 - t * kTile);
-        }
-      } else {
-        // next cell of the current shell
-        int dx = 0, dy = 0, dz = 0;
-        bool found = false;
+  for (int r = 1; r <= d.r_safe; ++r) {
+    const int side = 2 * r + 1;
+    const int ncube = side * side * side;
 #pragma unroll 1
-        while (!found) {
-          ci += 1;
-          if (ci >= ncube) break;
-          found = shell_offset(r, ci, dx, dy, dz);
-        }
-        if (!found) {
-          state = kWaitStop;
-          nt = 0;
-        } else {
-          const int jx = hx + dx, jy = hy + dy, jz = hz + dz;
-          const int cx = jx < 0 ? jx + nx : (jx >= nx ? jx - nx : jx);
-          const int cy = jy < 0 ? jy + ny : (jy >= ny ? jy - ny : jy);
-          const int cz = jz < 0 ? jz + nz : (jz >= nz ? jz - nz : jz);
-          const int na = jx < 0 ? -1 : (jx >= nx ? 1 : 0);
-          const int nb = jy < 0 ? -1 : (jy >= ny ? 1 : 0);
-          const int nc = jz < 0 ? -1 : (jz >= nz ? 1 : 0);
-          const int gq = cb + (cz * ny + cy) * nx + cx;
-          a0 = A.off[gq];
-          n2 = A.off[gq + 1] - a0;
-          nt = (n2 + kTile - 1) >> kTileShift;
-          tt = 0;
-          home = r == 1 && ci == 0;
-          start = home ? own_tile : 0;
-          t0 = A.tile_off[gq];
-          if (d.ortho) {
-            shx = static_cast<double>(na) * d.widths[0];
-            shy = static_cast<double>(nb) * d.widths[1];
-            shz = static_cast<double>(nc) * d.widths[2];
-          } else {
-            const double fa = na, fb = nb, fc = nc;
-            shx = fa * d.h[0] + fb * d.h[3] + fc * d.h[6];
-            shy = fa * d.h[1] + fb * d.h[4] + fc * d.h[7];
-            shz = fa * d.h[2] + fb * d.h[5] + fc * d.h[8];
-          }
-          if (nt > 1) {
-            const double lb = box_lb(px, py, pz, A.cbox[gq], shx, shy, shz, false);
-            if (lb > thr) nt = 0;
-          }
-        }
-      }
+    for (int idx = 0; idx < ncube; ++idx) {
+      int dx, dy, dz;
+      if (!shell_offset(r, idx, dx, dy, dz)) continue;
+      visit(dx, dy, dz, r == 1 && idx == 0);
     }
-    // --- one group of candidates
-    if (state == kWalk && cur < end) {
-      const int last = end - 1;
-      Cand cc[G];
-#pragma unroll
-      for (int u = 0; u < G; ++u) cc[u] = A.cand[min(cur + u, last)];
-      const int lim = end - cur;
-#pragma unroll
-      for (int u = 0; u < G; ++u) {
-        // q + 0.0 == q except -0.0, whose square is the same: no
-        // separate zero-shift path is needed for the d2 bits
-        const double ddx = cc[u].x + shx - px;
-        const double ddy = cc[u].y + shy - py;
-        const double ddz = cc[u].z + shz - pz;
-        const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        const int j = static_cast<int>(cc[u].id);
-        if (u < lim && d2 <= thr && j != me) {
-          bd[nbuf * kWave] = __double_as_longlong(d2);
-          bj[nbuf * kWave] = j;
-          nbuf += 1;
-        }
-      }
-      st_evals += lim < G ? lim : G;
-      cur += G;
-    }
-    // --- merges and stop tests, for all lanes at once
-    const unsigned long long waiting = __ballot(state == kWaitStop);
-    const unsigned long long walking = __ballot(state == kWalk);
-    if (__ballot(nbuf > KC - G) || __popcll(waiting) >= 16 ||
-        (walking == 0 && waiting != 0)) {
-      if (nbuf > 0) {
-        st_merges += 1;
-        batch_merge<KC>(h, bd, bj, nbuf, min_rej);
-        nbuf = 0;
-        thr = as_f64(h.d[KC - 1]);
-      }
-      if (state == kWaitStop) {
-        // stop test, src/knearest.rs:466-471
-        bool stop = false;
-        if (h.j[KC - 1] != kEmptyJ) {
-          const double bound = static_cast<double>(r) * cell_min;
-          stop = as_f64(h.d[KC - 1]) <= bound * bound;
-        }
-        if (stop) {
-          done_reach = r;
-          ok = true;
-          state = kDone;
-        } else if (r + 1 > r_safe) {
-          state = kDone;  // needs the wrapping stencil: exact walk
-        } else {
-          r += 1;
-          const int side = 2 * r + 1;
-          ncube = side * side * side;
-          ci = -1;
-          nt = 0;
-          tt = 0;
-          state = kWalk;
-        }
+    flush();
+    if (h.j[KC - 1] != kEmptyJ) {
+      const double bound = static_cast<double>(r) * d.cell_min;
+      if (as_f64(h.d[KC - 1]) <= bound * bound) {
+        done_reach = r;
+        break;
       }
     }
   }
-  if (A.stats != nullptr && lane == 0) {
-    // per-wave timeline (100 MHz wall clock) after the counters
+  if (A.stats != nullptr) {
     const long long wid = blockIdx.x * static_cast<long long>(W) + wv;
-    A.stats[kStatCount + 2 * wid] = t_begin;
-    A.stats[kStatCount + 2 * wid + 1] = wall_clock64();
-  }
-  if (A.stats != nullptr && s0 < nact) {
+    atomicMax(&A.stats[kStatCount + 2 * wid + 1], wall_clock64());
     stat_add(A.stats, kStatSparseSources, 1);
     stat_add(A.stats, kStatSparseTileTests, st_tests);
     stat_add(A.stats, kStatSparseTiles, st_tiles);
     stat_add(A.stats, kStatSparseEvals, st_evals);
     stat_add(A.stats, kStatSparseMerges, st_merges);
   }
-  if (s0 >= nact || cnt >= A.dense_min || d.kc != KC ||
-      (A.fflags[f] & kFrameNonFinite) != 0 || d.r_safe < 1) {
-    return;
-  }
   const size_t row = static_cast<size_t>(me) * k;
-  if (!ok || !top_emit<KC>(h, k, min_rej, d.out_nn, d.out_d2, row)) {
+  if (done_reach == 0 ||
+      !top_emit<KC>(h, k, min_rej, d.out_nn, d.out_d2, row)) {
     defer_source(s, A.defer_list, A.defer_count);
     return;
   }
