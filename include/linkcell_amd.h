@@ -13,6 +13,21 @@
 extern "C" {
 #endif
 
+/* One system of a batched search: its own points, cell, k and outputs
+ * (device pointers as in lc_gpu_knearest; out_d2, out_reach and mask may
+ * be NULL; cell_hint <= 0 selects the default). */
+typedef struct lc_gpu_frame {
+  const double *xyz;
+  size_t n;
+  struct lc_cell cell;
+  size_t k;
+  const int *mask;
+  double cell_hint;
+  int *out_nn;
+  double *out_d2;
+  int *out_reach;
+} lc_gpu_frame;
+
 /* Walk strategy of a workspace. */
 enum {
   LC_GPU_MODE_AUTO = 0,       /* canonical fast walk + exact-order rest */
@@ -37,9 +52,30 @@ int lc_gpu_set_mode(lc_gpu_workspace *ws, int mode);
 int lc_gpu_set_timing(lc_gpu_workspace *ws, int on);
 int lc_gpu_last_timing(lc_gpu_workspace *ws, double *ms7, int *deferred);
 
+/* Many uneven systems (different n, cell, mesh and k) in one pipeline:
+ * one launch per kernel for the whole batch. Errors name the frame. */
+int lc_gpu_knearest_batch(lc_gpu_workspace *ws, const lc_gpu_frame *frames,
+                          size_t n_frames, int wait);
+
+/* Capture the batch into a hipGraph held by the workspace (after one
+ * ordinary run that sizes every buffer); lc_gpu_graph_launch replays it on
+ * the workspace stream with whatever the frames' buffers hold then. */
+int lc_gpu_graph_capture(lc_gpu_workspace *ws, const lc_gpu_frame *frames,
+                         size_t n_frames);
+int lc_gpu_graph_launch(lc_gpu_workspace *ws, int wait);
+int lc_gpu_graph_reset(lc_gpu_workspace *ws);
+
+/* Time the batch under each walk split and keep the fastest for batches of
+ * its size bucket (largest k bucket, log2 point count, log2 mean cell
+ * occupancy). Results do not depend on the split. */
+int lc_gpu_autotune(lc_gpu_workspace *ws, const lc_gpu_frame *frames,
+                    size_t n_frames, int *best_dense_min, double *best_ms);
+
 /* Home-cell occupancy from which the wave-per-cell walk handles a cell
- * (the rest take the lane-per-source walk). Default 32, or the
- * LINKCELL_DENSE_MIN environment variable. */
+ * (the rest take the lane-per-source walk). 0 (the default) uses the
+ * autotuned value for the batch's size bucket, else every cell is walked
+ * lane per source. LINKCELL_DENSE_MIN sets it at workspace creation;
+ * LINKCELL_TPP=1 or 64 selects the all-lane or all-wave walk. */
 int lc_gpu_set_dense_min(lc_gpu_workspace *ws, int dense_min);
 
 /* Device work counters of the walk kernels (diagnostics). When on, each

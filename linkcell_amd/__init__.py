@@ -177,6 +177,50 @@ class Workspace:
             1 if wait else 0)
         self._check(rc)
 
+    @staticmethod
+    def _frames(frames):
+        """Build an ``lc_gpu_frame`` array from dicts with keys xyz, n,
+        cell (12 values), k, and optional mask, cell_hint, out_nn, out_d2,
+        out_reach; pointer values are device addresses (ints)."""
+        frames = list(frames)
+        arr = (_lib.lc_gpu_frame * max(1, len(frames)))()
+        for i, fr in enumerate(frames):
+            arr[i].xyz = fr["xyz"] or None
+            arr[i].n = int(fr["n"])
+            arr[i].cell = _lib.lc_cell.from12(fr["cell"])
+            arr[i].k = int(fr["k"])
+            arr[i].mask = fr.get("mask") or None
+            arr[i].cell_hint = float(fr.get("cell_hint") or 0.0)
+            arr[i].out_nn = fr["out_nn"] or None
+            arr[i].out_d2 = fr.get("out_d2") or None
+            arr[i].out_reach = fr.get("out_reach") or None
+        return arr, len(frames)
+
+    def knearest_batch(self, frames, wait: bool = True):
+        """Uneven systems in one pipeline (``lc_gpu_knearest_batch``)."""
+        arr, n = self._frames(frames)
+        self._check(self._lib.lc_gpu_knearest_batch(self._ws, arr, n, 1 if wait else 0))
+
+    def graph_capture(self, frames):
+        """Capture the batch into a replayable hipGraph."""
+        arr, n = self._frames(frames)
+        self._check(self._lib.lc_gpu_graph_capture(self._ws, arr, n))
+
+    def graph_launch(self, wait: bool = True):
+        self._check(self._lib.lc_gpu_graph_launch(self._ws, 1 if wait else 0))
+
+    def graph_reset(self):
+        self._check(self._lib.lc_gpu_graph_reset(self._ws))
+
+    def autotune(self, frames) -> tuple:
+        """Time each walk split on the batch; keep and return the fastest
+        as (dense_min, milliseconds)."""
+        arr, n = self._frames(frames)
+        best = C.c_int(0)
+        ms = C.c_double(0.0)
+        self._check(self._lib.lc_gpu_autotune(self._ws, arr, n, C.byref(best), C.byref(ms)))
+        return int(best.value), float(ms.value)
+
 
 _default_ws = None
 

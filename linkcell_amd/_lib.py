@@ -23,7 +23,8 @@ EXPORTS = (
     # include/linkcell_amd.h
     "lc_gpu_knearest_ex", "lc_gpu_set_mode", "lc_gpu_set_timing",
     "lc_gpu_last_timing", "lc_gpu_set_dense_min", "lc_gpu_set_stats",
-    "lc_gpu_last_stats",
+    "lc_gpu_last_stats", "lc_gpu_knearest_batch", "lc_gpu_graph_capture",
+    "lc_gpu_graph_launch", "lc_gpu_graph_reset", "lc_gpu_autotune",
 )
 
 
@@ -39,6 +40,16 @@ class lc_cell(C.Structure):
         if len(v) != 12:
             raise ValueError("lc_cell needs 12 values")
         return cls(*v)
+
+
+class lc_gpu_frame(C.Structure):
+    """Mirror of ``struct lc_gpu_frame``: one system of a batched search."""
+
+    _fields_ = [
+        ("xyz", C.c_void_p), ("n", C.c_size_t), ("cell", lc_cell),
+        ("k", C.c_size_t), ("mask", C.c_void_p), ("cell_hint", C.c_double),
+        ("out_nn", C.c_void_p), ("out_d2", C.c_void_p), ("out_reach", C.c_void_p),
+    ]
 
 
 _dp = C.POINTER(C.c_double)
@@ -75,6 +86,11 @@ _PROTOS = {
     "lc_gpu_set_dense_min": (C.c_int, [_vp, C.c_int]),
     "lc_gpu_set_stats": (C.c_int, [_vp, C.c_int]),
     "lc_gpu_last_stats": (C.c_int, [_vp, C.POINTER(C.c_ulonglong), C.c_int]),
+    "lc_gpu_knearest_batch": (C.c_int, [_vp, C.POINTER(lc_gpu_frame), _sz, C.c_int]),
+    "lc_gpu_graph_capture": (C.c_int, [_vp, C.POINTER(lc_gpu_frame), _sz]),
+    "lc_gpu_graph_launch": (C.c_int, [_vp, C.c_int]),
+    "lc_gpu_graph_reset": (C.c_int, [_vp]),
+    "lc_gpu_autotune": (C.c_int, [_vp, C.POINTER(lc_gpu_frame), _sz, _ip, _dp]),
 }
 
 _lib = None

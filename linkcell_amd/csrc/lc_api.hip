@@ -6,6 +6,7 @@
 //                                  through device memory
 //   lc_gpu_knearest_ex and the timing / mode extensions
 #include "../../include/linkcell_amd.h"
+#include "lc_engine.hpp"
 #include "../../include/linkcell_gpu.h"
 #include "../../include/linkcell_gpu.hpp"
 #include "lc_cell.hpp"
@@ -358,6 +359,117 @@ int lc_gpu_last_timing(lc_gpu_workspace* ws, double* ms7, int* deferred) {
     ms7[5] = t.exact;
     ms7[6] = t.total;
     if (deferred) *deferred = t.deferred;
+  });
+}
+
+}  // extern "C"
+
+namespace {
+
+// Validate and convert a batch (src/gpu/knearest.cpp:711-762 per frame).
+std::vector<lcamd::FrameSpec> batch_specs(const lc_gpu_frame* frames,
+                                          size_t n_frames) {
+  using linkcell::Error;
+  if (!device_present()) throw Error(kNoDevice);
+  if (frames == nullptr) throw Error("null pointer");
+  if (n_frames == 0) throw Error("no points");
+  std::vector<lcamd::FrameSpec> out(n_frames);
+  size_t total = 0;
+  for (size_t f = 0; f < n_frames; ++f) {
+    const lc_gpu_frame& fr = frames[f];
+    const std::string tag = "frame " + std::to_string(f) + ": ";
+    if (fr.n == 0) throw Error(tag + "no points");
+    if (fr.k == 0) throw Error(tag + "k must be at least 1");
+    if (fr.xyz == nullptr || fr.out_nn == nullptr) throw Error(tag + "null pointer");
+    if (fr.n >= static_cast<size_t>(INT_MAX) || fr.k >= static_cast<size_t>(INT_MAX) ||
+        fr.n > SIZE_MAX / fr.k) {
+      throw Error(tag + "n * k overflows");
+    }
+    total += fr.n;
+    if (total >= static_cast<size_t>(INT_MAX) / 2) {
+      throw Error("batch exceeds 2^30 points");
+    }
+    const lc_cell& cb = fr.cell;
+    const double a[3] = {cb.ax, cb.ay, cb.az}, b[3] = {cb.bx, cb.by, cb.bz},
+                 c[3] = {cb.cx, cb.cy, cb.cz}, o[3] = {cb.ox, cb.oy, cb.oz};
+    lcamd::FrameSpec& s = out[f];
+    if (lcamd::cell_from_vectors(a, b, c, o, s.geom) != lcamd::kOk) {
+      throw Error(tag + "bad cell");
+    }
+    if (lcamd::plan_mesh(s.geom, hint_option(fr.cell_hint), s.plan) != lcamd::kOk) {
+      throw Error(tag + "too many cells");
+    }
+    s.xyz = fr.xyz;
+    s.mask = fr.mask;
+    s.out_nn = fr.out_nn;
+    s.out_d2 = fr.out_d2;
+    s.out_reach = fr.out_reach;
+    s.n = static_cast<int>(fr.n);
+    s.k = static_cast<int>(fr.k);
+  }
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lc_gpu_knearest_batch(lc_gpu_workspace* ws, const lc_gpu_frame* frames,
+                          size_t n_frames, int wait) {
+  if (ws == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    const auto specs = batch_specs(frames, n_frames);
+    try {
+      ws->ws.impl()->engine.run(specs, wait != 0);
+    } catch (const std::runtime_error& e) {
+      throw linkcell::Error(e.what());
+    }
+  });
+}
+
+int lc_gpu_graph_capture(lc_gpu_workspace* ws, const lc_gpu_frame* frames,
+                         size_t n_frames) {
+  if (ws == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    const auto specs = batch_specs(frames, n_frames);
+    ws->ws.impl()->engine.capture(specs);
+  });
+}
+
+int lc_gpu_graph_launch(lc_gpu_workspace* ws, int wait) {
+  if (ws == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] { ws->ws.impl()->engine.replay(wait != 0); });
+}
+
+int lc_gpu_graph_reset(lc_gpu_workspace* ws) {
+  if (ws == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] { ws->ws.impl()->engine.reset_graph(); });
+}
+
+int lc_gpu_autotune(lc_gpu_workspace* ws, const lc_gpu_frame* frames,
+                    size_t n_frames, int* best_dense_min, double* best_ms) {
+  if (ws == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    const auto specs = batch_specs(frames, n_frames);
+    float ms = 0.0f;
+    const int best = ws->ws.impl()->engine.autotune(specs, &ms);
+    if (best_dense_min) *best_dense_min = best;
+    if (best_ms) *best_ms = ms;
   });
 }
 

@@ -62,6 +62,7 @@ constexpr int kTile = 16;      // points per bounding-box tile
 constexpr int kTileShift = 4;
 constexpr int kChunkShift = 6;  // sources per dense work item (one wave)
 constexpr int kPad = 16;        // slack elements after SoA arrays (group reads)
+constexpr int kNeverDense = 1 << 30;  // dense_min that keeps every cell sparse
 
 // Cell-ordered candidate record: folded coordinates and the point's index
 // within its frame (64-bit, so a record is 32 bytes: one s_load_dwordx8
@@ -1281,7 +1282,15 @@ Engine::Engine() {
   // LINKCELL_BLOCK (src/gpu/knearest.cpp:948-960)
   if (const char* v = std::getenv("LINKCELL_DENSE_MIN")) {
     const int x = std::atoi(v);
-    if (x >= 1) dense_min_ = x;
+    if (x >= 1) dense_min_override_ = x;
+  }
+  // LINKCELL_TPP keeps its meaning of "threads per source": 1 sends every
+  // source to the lane-per-source walk, 64 every cell to the wave-per-cell
+  // walk.
+  if (const char* v = std::getenv("LINKCELL_TPP")) {
+    const int x = std::atoi(v);
+    if (x == 1) dense_min_override_ = kNeverDense;
+    if (x >= 64) dense_min_override_ = 1;
   }
 }
 
@@ -1290,6 +1299,8 @@ Engine::~Engine() {
     (void)hipStreamSynchronize(stream_);
   }
   free_all();
+  if (graph_exec_ != nullptr) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_staging_ != nullptr) (void)hipHostFree(graph_staging_);
   if (staging_ != nullptr) (void)hipHostFree(staging_);
   if (staging_ev_ != nullptr) (void)hipEventDestroy(staging_ev_);
   for (auto& e : ev_) {
@@ -1312,6 +1323,10 @@ void Engine::sync() {
 
 void Engine::grow(Buf& b, size_t bytes) {
   if (bytes <= b.bytes && b.p != nullptr) return;
+  if (capturing_) {
+    throw std::runtime_error(
+        "graph capture needs buffers sized by a prior run of the same batch");
+  }
   if (b.p != nullptr) {
     // a previous call may still read the old buffer
     check(hipStreamSynchronize(stream()), "stream wait");
@@ -1450,22 +1465,31 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   // copy has consumed it
   const size_t need = sizeof(FrameDesc) * static_cast<size_t>(F) +
                       2 * sizeof(long long) * (static_cast<size_t>(F) + 1);
-  if (staging_ev_ == nullptr) {
-    check(hipEventCreateWithFlags(&staging_ev_, hipEventDisableTiming),
-          "hipEventCreate");
+  char* sp = nullptr;
+  if (capturing_) {
+    // the graph's memcpy nodes read this buffer at every replay
+    if (need > graph_staging_bytes_) {
+      throw std::runtime_error("graph staging buffer too small");
+    }
+    sp = static_cast<char*>(graph_staging_);
+  } else {
+    if (staging_ev_ == nullptr) {
+      check(hipEventCreateWithFlags(&staging_ev_, hipEventDisableTiming),
+            "hipEventCreate");
+    }
+    if (staging_pending_) {
+      check(hipEventSynchronize(staging_ev_), "staging wait");
+      staging_pending_ = false;
+    }
+    if (need > staging_bytes_) {
+      if (staging_ != nullptr) check(hipHostFree(staging_), "hipHostFree");
+      staging_ = nullptr;
+      check(hipHostMalloc(&staging_, need, hipHostMallocDefault),
+            "hipHostMalloc");
+      staging_bytes_ = need;
+    }
+    sp = static_cast<char*>(staging_);
   }
-  if (staging_pending_) {
-    check(hipEventSynchronize(staging_ev_), "staging wait");
-    staging_pending_ = false;
-  }
-  if (need > staging_bytes_) {
-    if (staging_ != nullptr) check(hipHostFree(staging_), "hipHostFree");
-    staging_ = nullptr;
-    check(hipHostMalloc(&staging_, need, hipHostMallocDefault),
-          "hipHostMalloc");
-    staging_bytes_ = need;
-  }
-  char* sp = static_cast<char*>(staging_);
   const size_t dbytes = sizeof(FrameDesc) * static_cast<size_t>(F);
   const size_t bbytes = sizeof(long long) * (static_cast<size_t>(F) + 1);
   std::memcpy(sp, desc.data(), dbytes);
@@ -1478,10 +1502,13 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   check(hipMemcpyAsync(fcb_.p, sp + dbytes + bbytes, bbytes,
                        hipMemcpyHostToDevice, st),
         "HtoD fcb");
-  check(hipEventRecord(staging_ev_, st), "event record");
-  staging_pending_ = true;
+  if (!capturing_) {
+    check(hipEventRecord(staging_ev_, st), "event record");
+    staging_pending_ = true;
+  }
+  const bool timing = timing_ && !capturing_;
 
-  if (timing_) {
+  if (timing) {
     for (auto& e : ev_) {
       if (e == nullptr) check(hipEventCreate(&e), "hipEventCreate");
     }
@@ -1507,7 +1534,7 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
       dfr, dfpt, F, ntot, as<int>(count_.p), as<int4>(cellslot_.p),
       as<int>(flags_.p));
   check(hipGetLastError(), "fold_count launch");
-  if (timing_) check(hipEventRecord(ev_[1], st), "event record");
+  if (timing) check(hipEventRecord(ev_[1], st), "event record");
 
   k_scan<<<static_cast<unsigned>(tiles), kScanThreads, 0, st>>>(
       as<int>(count_.p), ctot, kTileShift, as<int>(off_.p),
@@ -1516,7 +1543,7 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
       as<int>(count_.p), ctot, kChunkShift, nullptr, as<int>(item_off_.p),
       status + tiles, counters + 2);
   check(hipGetLastError(), "scan launch");
-  if (timing_) check(hipEventRecord(ev_[2], st), "event record");
+  if (timing) check(hipEventRecord(ev_[2], st), "event record");
 
   k_emit_items<<<blocks_for(ctot, 256), 256, 0, st>>>(
       as<int>(count_.p), as<int>(item_off_.p), ctot, as<int>(items_.p));
@@ -1524,7 +1551,7 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
       as<int4>(cellslot_.p), dfpt, F, ntot, as<int>(off_.p),
       as<long long>(tmp_key_.p));
   check(hipGetLastError(), "scatter launch");
-  if (timing_) check(hipEventRecord(ev_[3], st), "event record");
+  if (timing) check(hipEventRecord(ev_[3], st), "event record");
 
   const unsigned item_blocks = blocks_for(max_items * kWave, 256);
   k_cell_sort<<<item_blocks, 256, 0, st>>>(
@@ -1538,7 +1565,7 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
       as<int>(off_.p), as<int>(tile_off_.p), ctot, as<Box>(tbox_.p),
       as<Box>(cbox_.p));
   check(hipGetLastError(), "cell_sort launch");
-  if (timing_) check(hipEventRecord(ev_[4], st), "event record");
+  if (timing) check(hipEventRecord(ev_[4], st), "event record");
 
   WalkArgs wa;
   wa.fr = dfr;
@@ -1554,11 +1581,11 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   wa.ncells = ctot;
   wa.tbox = as<Box>(tbox_.p);
   wa.cbox = as<Box>(cbox_.p);
-  wa.dense_min = dense_min_;
+  wa.dense_min = dense_min_for(frames);
   wa.defer_list = as<int>(defer_.p);
   wa.defer_count = counters + 1;
   wa.stats = nullptr;
-  if (stats_on_) {
+  if (stats_on_ && !capturing_) {
     const long long nstat = kStatCount + 2 * (ntot / kWave + 1);
     grow(stats_, sizeof(unsigned long long) * static_cast<size_t>(nstat));
     k_zero_u32<<<blocks_for(2 * nstat, 256), 256, 0, st>>>(
@@ -1591,19 +1618,138 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
         as<int>(defer_.p), counters + 1);
     check(hipGetLastError(), "route launch");
   }
-  if (timing_) check(hipEventRecord(ev_[5], st), "event record");
+  if (timing) check(hipEventRecord(ev_[5], st), "event record");
   k_walk_exact<<<static_cast<unsigned>(exact_threads / 64), 64, 0, st>>>(
       dfr, dfcb, F, as<Cand>(cand_.p), as<int>(scell_.p), as<int>(off_.p),
       as<int>(byidx_.p),
       as<int>(defer_.p), counters + 1, as<double>(heap_d2_.p),
       as<int>(heap_idx_.p));
   check(hipGetLastError(), "exact walk launch");
-  if (timing_) {
+  if (timing) {
     check(hipEventRecord(ev_[6], st), "event record");
     check(hipEventRecord(ev_[7], st), "event record");
     timings_pending_ = true;
   }
+  if (wait && !capturing_) sync();
+}
+
+namespace {
+
+// Size bucket of a batch for the tuned table: largest register-heap
+// bucket, log2 of the point count, log2 of the mean cell occupancy.
+std::string bucket_key(const std::vector<FrameSpec>& frames) {
+  long long n = 0, c = 0;
+  int kc = 0;
+  for (const FrameSpec& f : frames) {
+    n += f.n;
+    c += f.plan.ncell;
+    kc = std::max(kc, fast_bucket(f.k));
+  }
+  int ln = 0, lo = 0;
+  while ((1ll << (ln + 1)) <= n) ++ln;
+  const long long occ = c > 0 ? n / c : n;
+  while ((1ll << (lo + 1)) <= occ) ++lo;
+  return "kc" + std::to_string(kc) + ":n2^" + std::to_string(ln) + ":occ2^" +
+         std::to_string(lo);
+}
+
+}  // namespace
+
+int Engine::dense_min_for(const std::vector<FrameSpec>& frames) const {
+  if (dense_min_override_ > 0) return dense_min_override_;
+  const std::string key = bucket_key(frames);
+  for (const auto& e : tuned_) {
+    if (e.first == key) return e.second;
+  }
+  return kNeverDense;
+}
+
+void Engine::capture(const std::vector<FrameSpec>& frames) {
+  reset_graph();
+  run(frames, true);  // sizes every buffer
+  const size_t need = sizeof(FrameDesc) * frames.size() +
+                      2 * sizeof(long long) * (frames.size() + 1);
+  if (need > graph_staging_bytes_) {
+    if (graph_staging_ != nullptr) check(hipHostFree(graph_staging_), "hipHostFree");
+    graph_staging_ = nullptr;
+    check(hipHostMalloc(&graph_staging_, need, hipHostMallocDefault),
+          "hipHostMalloc");
+    graph_staging_bytes_ = need;
+  }
+  hipStream_t st = stream();
+  hipGraph_t graph = nullptr;
+  check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal),
+        "hipStreamBeginCapture");
+  capturing_ = true;
+  try {
+    run(frames, false);
+  } catch (...) {
+    capturing_ = false;
+    (void)hipStreamEndCapture(st, &graph);
+    if (graph != nullptr) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  capturing_ = false;
+  check(hipStreamEndCapture(st, &graph), "hipStreamEndCapture");
+  const hipError_t e = hipGraphInstantiate(&graph_exec_, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  check(e, "hipGraphInstantiate");
+}
+
+void Engine::replay(bool wait) {
+  if (graph_exec_ == nullptr) throw std::runtime_error("no captured graph");
+  check(hipGraphLaunch(graph_exec_, stream()), "hipGraphLaunch");
   if (wait) sync();
+}
+
+void Engine::reset_graph() {
+  if (graph_exec_ != nullptr) {
+    sync();
+    (void)hipGraphExecDestroy(graph_exec_);
+    graph_exec_ = nullptr;
+  }
+}
+
+int Engine::autotune(const std::vector<FrameSpec>& frames, float* best_ms) {
+  static const int kCands[] = {1, 16, 32, 64, 128, kNeverDense};
+  const int keep_override = dense_min_override_;
+  const bool keep_timing = timing_;
+  int best = kNeverDense;
+  float best_t = 0.0f;
+  timing_ = true;
+  try {
+    for (int cand : kCands) {
+      dense_min_override_ = cand;
+      run(frames, true);  // warm-up at this split
+      float t[3];
+      for (float& x : t) {
+        run(frames, true);
+        x = last_timings().total;
+      }
+      std::sort(t, t + 3);
+      if (best_t == 0.0f || t[1] < best_t) {
+        best_t = t[1];
+        best = cand;
+      }
+    }
+  } catch (...) {
+    dense_min_override_ = keep_override;
+    timing_ = keep_timing;
+    throw;
+  }
+  dense_min_override_ = keep_override;
+  timing_ = keep_timing;
+  const std::string key = bucket_key(frames);
+  bool set = false;
+  for (auto& e : tuned_) {
+    if (e.first == key) {
+      e.second = best;
+      set = true;
+    }
+  }
+  if (!set) tuned_.emplace_back(key, best);
+  if (best_ms) *best_ms = best_t;
+  return best;
 }
 
 }  // namespace lcamd

@@ -13,6 +13,7 @@
 
 #include <cstddef>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace lcamd {
@@ -78,13 +79,27 @@ class Engine {
   // Runs a batch. Throws std::runtime_error on HIP failures.
   void run(const std::vector<FrameSpec>& frames, bool wait);
 
+  // hipGraph: run the batch once (sizing every buffer), then capture the
+  // same batch into a graph that replay() relaunches. The frames' device
+  // pointers and sizes are baked in; callers refill the same buffers.
+  void capture(const std::vector<FrameSpec>& frames);
+  void replay(bool wait);
+  bool has_graph() const { return graph_exec_ != nullptr; }
+  void reset_graph();
+
+  // Times the batch under each walk split (see set_dense_min) and keeps
+  // the fastest for the batch's size bucket; later runs of that bucket use
+  // it. Returns the best split and its pipeline time in milliseconds.
+  int autotune(const std::vector<FrameSpec>& frames, float* best_ms);
+
   void set_timing(bool on) { timing_ = on; }
   const Timings& last_timings();
   void set_mode(WalkMode m) { mode_ = m; }
   // Home cells with at least this many sources take the wave-per-cell
   // walk, the rest the lane-per-source walk.
-  void set_dense_min(int v) { dense_min_ = v < 1 ? 1 : v; }
-  int dense_min() const { return dense_min_; }
+  // An explicit value overrides the tuned table; 0 returns to it.
+  void set_dense_min(int v) { dense_min_override_ = v < 0 ? 0 : v; }
+  int dense_min_for(const std::vector<FrameSpec>& frames) const;
   // Device work counters of the walks (diagnostics).
   void set_stats(bool on) { stats_on_ = on; }
   void read_stats(unsigned long long* out, int n);
@@ -109,7 +124,12 @@ class Engine {
   bool timing_ = false;
   bool timings_pending_ = false;
   WalkMode mode_ = WalkMode::kAuto;
-  int dense_min_ = 32;
+  int dense_min_override_ = 0;
+  bool capturing_ = false;
+  hipGraphExec_t graph_exec_ = nullptr;
+  void* graph_staging_ = nullptr;
+  size_t graph_staging_bytes_ = 0;
+  std::vector<std::pair<std::string, int>> tuned_;
   bool stats_on_ = false;
   long long stats_n_ = 0;
   hipEvent_t ev_[8] = {};
