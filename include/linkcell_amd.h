@@ -71,12 +71,37 @@ int lc_gpu_graph_reset(lc_gpu_workspace *ws);
 int lc_gpu_autotune(lc_gpu_workspace *ws, const lc_gpu_frame *frames,
                     size_t n_frames, int *best_dense_min, double *best_ms);
 
-/* Home-cell occupancy from which the wave-per-cell walk handles a cell
- * (the rest take the lane-per-source walk). 0 (the default) uses the
- * autotuned value for the batch's size bucket, else every cell is walked
- * lane per source. LINKCELL_DENSE_MIN sets it at workspace creation;
- * LINKCELL_TPP=1 or 64 selects the all-lane or all-wave walk. */
+/* Walk split. LC_GPU_ROW_WALK (-1): one 16-lane row per 16-source tile
+ * of a home cell. A positive value: home cells with at least that many
+ * sources take the wave-per-cell walk, the rest the lane-per-source walk.
+ * 0 (the default) uses the autotuned split for the batch's size bucket,
+ * else the row walk. LINKCELL_DENSE_MIN sets a positive split at
+ * workspace creation; LINKCELL_TPP=1, 16 or 64 selects the all-lane, row
+ * or all-wave walk. */
+#define LC_GPU_ROW_WALK (-1)
 int lc_gpu_set_dense_min(lc_gpu_workspace *ws, int dense_min);
+
+/* Order in which the lane-per-source walk takes its sources. Only the
+ * order changes: each source runs the same walk and stop test, so results
+ * are identical under every schedule.
+ *   LC_GPU_SCHED_CELL     cell order (the default)
+ *   LC_GPU_SCHED_LEARNED  grouped by the depth the workspace's trained
+ *                         predictor gives each home cell (cell order until
+ *                         one is trained or loaded)
+ *   LC_GPU_SCHED_GIVEN    grouped by the per-cell depths set with
+ *                         lc_gpu_sched_set_depth
+ * Groups run deepest first, so that long waves start early. */
+#define LC_GPU_SCHED_CELL 0
+#define LC_GPU_SCHED_LEARNED 1
+#define LC_GPU_SCHED_GIVEN 2
+int lc_gpu_set_schedule(lc_gpu_workspace *ws, int mode);
+
+/* Depth per home cell for LC_GPU_SCHED_GIVEN: a device array of n_cells
+ * ints, one per cell of the batch (frames in order, cells in the
+ * reference's (cz*ny + cy)*nx + cx numbering). Kept by pointer; it must
+ * stay valid while searches use it. */
+int lc_gpu_sched_set_depth(lc_gpu_workspace *ws, const int *depth,
+                           size_t n_cells);
 
 /* Device work counters of the walk kernels (diagnostics). When on, each
  * search accumulates them; lc_gpu_last_stats waits for the last search

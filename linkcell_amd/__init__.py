@@ -139,13 +139,25 @@ class Workspace:
     def set_dense_min(self, v: int):
         self._check(self._lib.lc_gpu_set_dense_min(self._ws, int(v)))
 
+    SCHED_CELL, SCHED_LEARNED, SCHED_GIVEN = 0, 1, 2
+
+    def set_schedule(self, mode: int):
+        """Order of the lane-per-source walk (results never change)."""
+        self._check(self._lib.lc_gpu_set_schedule(self._ws, int(mode)))
+
+    def sched_set_depth(self, depth_ptr: int, n_cells: int):
+        """Per-cell depths (device int32 array) for SCHED_GIVEN."""
+        self._check(self._lib.lc_gpu_sched_set_depth(
+            self._ws, C.c_void_p(depth_ptr or None), int(n_cells)))
+
     def set_stats(self, on: bool):
         self._check(self._lib.lc_gpu_set_stats(self._ws, 1 if on else 0))
 
     STAT_NAMES = ("dense_waves", "dense_tile_tests", "dense_tiles", "dense_steps",
                   "dense_merges", "dense_appends", "sparse_sources",
                   "sparse_tile_tests", "sparse_tiles", "sparse_evals",
-                  "sparse_merges")
+                  "sparse_merges", "row_sources", "row_tile_tests", "row_tiles",
+                  "row_merges", "row_rounds", "row_cells", "row_refines")
 
     def last_stats(self) -> dict:
         n = len(self.STAT_NAMES)

@@ -493,6 +493,32 @@ int lc_gpu_set_stats(lc_gpu_workspace* ws, int on) {
   return 0;
 }
 
+int lc_gpu_set_schedule(lc_gpu_workspace* ws, int mode) {
+  if (ws == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  if (mode != LC_GPU_SCHED_CELL && mode != LC_GPU_SCHED_LEARNED &&
+      mode != LC_GPU_SCHED_GIVEN) {
+    set_gpu_err("unknown schedule");
+    return 1;
+  }
+  ws->ws.impl()->engine.set_schedule(mode);
+  set_gpu_err(nullptr);
+  return 0;
+}
+
+int lc_gpu_sched_set_depth(lc_gpu_workspace* ws, const int* depth,
+                           size_t n_cells) {
+  if (ws == nullptr || (depth == nullptr && n_cells != 0)) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  ws->ws.impl()->engine.set_given_depth(depth, static_cast<long long>(n_cells));
+  set_gpu_err(nullptr);
+  return 0;
+}
+
 int lc_gpu_last_stats(lc_gpu_workspace* ws, unsigned long long* out, int n) {
   if (ws == nullptr || out == nullptr) {
     set_gpu_err("null pointer");

@@ -63,6 +63,7 @@ constexpr int kTileShift = 4;
 constexpr int kChunkShift = 6;  // sources per dense work item (one wave)
 constexpr int kPad = 16;        // slack elements after SoA arrays (group reads)
 constexpr int kNeverDense = 1 << 30;  // dense_min that keeps every cell sparse
+constexpr int kRowWalk = -1;          // split value selecting the row walk
 
 // Cell-ordered candidate record: folded coordinates and the point's index
 // within its frame (64-bit, so a record is 32 bytes: one s_load_dwordx8
@@ -309,7 +310,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(
 
 __global__ __launch_bounds__(256) void k_emit_items(
     const int* __restrict__ count, const int* __restrict__ item_off,
-    long long ncells, int* __restrict__ items) {
+    const int* __restrict__ tile_off, long long ncells,
+    int* __restrict__ items, int* __restrict__ tile_cell) {
   const long long g = blockIdx.x * static_cast<long long>(blockDim.x) +
                       threadIdx.x;
   if (g >= ncells) return;
@@ -318,6 +320,9 @@ __global__ __launch_bounds__(256) void k_emit_items(
   const int nch = (c + (1 << kChunkShift) - 1) >> kChunkShift;
   const int b = item_off[g];
   for (int q = 0; q < nch; ++q) items[b + q] = static_cast<int>(g);
+  const int nt = (c + kTile - 1) >> kTileShift;
+  const int tb = tile_off[g];
+  for (int q = 0; q < nt; ++q) tile_cell[tb + q] = static_cast<int>(g);
 }
 
 __global__ __launch_bounds__(256) void k_scatter(
@@ -541,6 +546,37 @@ __device__ __forceinline__ void batch_merge(Top<KC>& h, const long long* bd,
   }
 }
 
+// Merge n buffered candidates into the top-k one at a time: each is
+// inserted into the ascending register array (evicting the largest) or
+// rejected. Few registers beyond the heap itself; the same kept set and
+// the same min_rej as a batch merge.
+template <int KC>
+__device__ __forceinline__ void insert_merge(Top<KC>& h, const long long* bd,
+                                             const int* bj, int n,
+                                             double& min_rej) {
+#pragma unroll 1
+  for (int e = 0; e < n; ++e) {
+    const long long x = bd[e * kWave];
+    const int xj = bj[e * kWave];
+    const long long top = h.d[KC - 1];
+    if (x < top) {
+      if (h.j[KC - 1] != kEmptyJ) min_rej = fmin(min_rej, as_f64(top));
+#pragma unroll
+      for (int t = KC - 1; t > 0; --t) {
+        const bool up = h.d[t - 1] > x;
+        const bool here = !up && h.d[t] > x;
+        h.d[t] = up ? h.d[t - 1] : (here ? x : h.d[t]);
+        h.j[t] = up ? h.j[t - 1] : (here ? xj : h.j[t]);
+      }
+      const bool here0 = h.d[0] > x;
+      h.d[0] = here0 ? x : h.d[0];
+      h.j[0] = here0 ? xj : h.j[0];
+    } else {
+      min_rej = fmin(min_rej, as_f64(x));
+    }
+  }
+}
+
 // Finish a source with a full heap: the row is exact unless a rejected or
 // evicted d2 equals the k-th (a boundary tie, where the reference keeps
 // whichever index it met first). Equal d2 inside the row is ordered by
@@ -610,6 +646,15 @@ __device__ __forceinline__ double box_lb(double px, double py, double pz,
   return gx * gx + gy * gy + gz * gz;
 }
 
+// Depth schedule buckets: register-heap bucket (4 values) x predicted
+// depth 1..kSchedDepths, deepest first inside each heap bucket.
+constexpr int kSchedDepths = 4;
+constexpr int kSchedBuckets = 4 * kSchedDepths;
+
+__host__ __device__ constexpr int sched_kc_index(int kc) {
+  return kc == 4 ? 0 : (kc == 8 ? 1 : (kc == 16 ? 2 : 3));
+}
+
 struct WalkArgs {
   const FrameDesc* fr;
   const long long* fcb;
@@ -629,6 +674,11 @@ struct WalkArgs {
   int* defer_count;
   // optional work counters (null = off): see kStat* indices
   unsigned long long* stats;
+  // optional depth schedule of the lane-per-source walk (null = cell
+  // order): source slots grouped by (heap bucket, predicted depth)
+  const int* sched;
+  const int* sched_count;  // kSchedBuckets run lengths
+  const int* tile_cell;    // home cell of every 16-slot source tile
 };
 
 enum {
@@ -643,7 +693,14 @@ enum {
   kStatSparseTiles,
   kStatSparseEvals,
   kStatSparseMerges,
-  kStatCount = 16
+  kStatRowSources,
+  kStatRowTileTests,
+  kStatRowTiles,
+  kStatRowMerges,
+  kStatRowRounds,   // wave-level tile rounds of the row walk
+  kStatRowCells,    // wave-level candidate-cell visits of the row walk
+  kStatRowRefines,  // row-level exact tile refinements
+  kStatCount = 24
 };
 
 __device__ __forceinline__ void stat_add(unsigned long long* st, int i,
@@ -910,15 +967,27 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
   const int wv = threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   const unsigned long long t_begin = wall_clock64();
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nact = A.off[A.ncells];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (A.stats != nullptr && lane == 0) {
     // per-wave timeline (100 MHz wall clock) after the counters; the
     // end stamp is written by the wave's last lane to finish
     const long long wid = blockIdx.x * static_cast<long long>(W) + wv;
     A.stats[kStatCount + 2 * wid] = t_begin;
   }
-  if (s >= nact) return;
+  int s = i;
+  if (A.sched != nullptr) {
+    // this heap bucket's run of the depth schedule
+    const int q0 = sched_kc_index(KC) * kSchedDepths;
+    int lo = 0, len = 0;
+    for (int q = 0; q < q0 + kSchedDepths; ++q) {
+      const int c = A.sched_count[q];
+      if (q < q0) lo += c; else len += c;
+    }
+    if (i >= len) return;
+    s = A.sched[lo + i];
+  } else if (i >= A.off[A.ncells]) {
+    return;
+  }
   const int g = A.scell[s];
   const int o0 = A.off[g];
   const int cnt = A.off[g + 1] - o0;
@@ -1093,6 +1162,500 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
   if (d.out_reach) d.out_reach[me] = done_reach;
 }
 
+// ------------------------------------------------------------- row walk
+
+// Lane I of each 16-lane row, broadcast to the whole row (DPP
+// row_newbcast: one VALU move per 64-bit value, no LDS traffic).
+template <int I>
+__device__ __forceinline__ double row_bcast(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const long long zero = 0;
+  const long long r =
+      __builtin_amdgcn_update_dpp(zero, b, 0x150 + I, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, r);
+}
+template <int I>
+__device__ __forceinline__ int row_bcast(int v) {
+  const int zero = 0;
+  return __builtin_amdgcn_update_dpp(zero, v, 0x150 + I, 0xf, 0xf, true);
+}
+
+// Lower bound of d2 between any point of the source box [sl, sh] and any
+// point of box b shifted by s, with the walk's rounded operations (see
+// box_lb: fl(fl(q + s) - p) is monotone in q and in p, so the bound never
+// exceeds a d2 the walk computes for a pair inside the boxes).
+__device__ __forceinline__ double hull_lb(const double (&sl)[3],
+                                          const double (&sh)[3], const Box& b,
+                                          double shx, double shy, double shz) {
+  const double lx = b.lx + shx, hx = b.hx + shx;
+  const double ly = b.ly + shy, hy = b.hy + shy;
+  const double lz = b.lz + shz, hz = b.hz + shz;
+  const double gx = fmax(fmax(lx - sh[0], sl[0] - hx), 0.0);
+  const double gy = fmax(fmax(ly - sh[1], sl[1] - hy), 0.0);
+  const double gz = fmax(fmax(lz - sh[2], sl[2] - hz), 0.0);
+  return gx * gx + gy * gy + gz * gz;
+}
+
+// i-th cell of Chebyshev shell r: shell 1 from the near-first table, the
+// surface of shell r > 1 in lexicographic (dx, dy, dz) order.
+__device__ __forceinline__ int shell_cells(int r) {
+  const int s = 2 * r + 1;
+  return r == 1 ? 27 : s * s * s - (s - 2) * (s - 2) * (s - 2);
+}
+__device__ __forceinline__ void surface_cell(int r, int i, int& dx, int& dy,
+                                             int& dz) {
+  if (r == 1) {
+    dx = kNear27[i][0];
+    dy = kNear27[i][1];
+    dz = kNear27[i][2];
+    return;
+  }
+  const int s = 2 * r + 1;
+  const int face = s * s;
+  const int ring = 4 * s - 4;
+  if (i < face) {
+    dx = -r;
+    dy = -r + i / s;
+    dz = -r + i % s;
+    return;
+  }
+  i -= face;
+  if (i < (s - 2) * ring) {
+    dx = -r + 1 + i / ring;
+    int q = i % ring;
+    if (q < s) {
+      dy = -r;
+      dz = -r + q;
+    } else if (q < s + 2 * (s - 2)) {
+      q -= s;
+      dy = -r + 1 + q / 2;
+      dz = (q & 1) ? r : -r;
+    } else {
+      q -= s + 2 * (s - 2);
+      dy = r;
+      dz = -r + q;
+    }
+    return;
+  }
+  i -= (s - 2) * ring;
+  dx = r;
+  dy = -r + i / s;
+  dz = -r + i % s;
+}
+
+// First wave of frame f in the row walk: frames start on wave boundaries
+// so that every wave's frame data is uniform. ceil(t/4) + f leaves at
+// most one spare wave per frame and needs no scan.
+__device__ __forceinline__ int row_wave0(const int* tile_off,
+                                         const long long* fcb, int f) {
+  return (tile_off[fcb[f]] + 3) / 4 + f;
+}
+
+// One 16-lane row per source tile (16 Morton-consecutive sources of one
+// home cell), four rows per wave, every wave inside one frame. Per shell,
+// a row lists the candidate tiles some of its sources still need: 16 cells
+// (one per lane) get their headers at once, then 16 of their tiles (one per
+// lane) get a bounding-box test against the row's sources and, when that
+// passes, an exact per-source test (each source broadcast to the row). The
+// list, in LDS, is then drained tile by tile: each lane loads one
+// candidate of the tile, adds the lattice shift, and the 16 candidates
+// reach the row's lanes through DPP broadcasts, while the next tile's loads
+// are in flight. The row's own tile, already in registers, and its Morton
+// neighbour open the walk, so the tests start from a finite k-th distance.
+template <int KC>
+__global__ __launch_bounds__(256) void k_walk_rows(WalkArgs A) {
+  constexpr int W = 4;   // waves per block
+  constexpr int BC = 8;  // buffered candidates per lane between merges
+  constexpr int LC = 48; // tile list entries per row
+  __shared__ long long s_bd[W][BC][kWave];
+  __shared__ int s_bj[W][BC][kWave];
+  __shared__ int s_slot[W][4][LC];
+  __shared__ int s_meta[W][4][LC];
+  const int wv = uni(static_cast<int>(threadIdx.x) / kWave);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int li = lane & 15;
+  const int rw = lane >> 4;
+  const int rbase = lane & 48;
+  const int wg = uni(static_cast<int>(blockIdx.x) * W + wv);
+  // the frame whose waves contain wg
+  int f = 0;
+  {
+    int lo = 0, hi = A.F - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (row_wave0(A.tile_off, A.fcb, mid) <= wg) lo = mid;
+      else hi = mid - 1;
+    }
+    f = uni(lo);
+  }
+  const FrameDesc& d = A.fr[f];
+  if (d.kc != KC) return;
+  const int tb = A.tile_off[A.fcb[f]];
+  const int nt_frame = A.tile_off[A.fcb[f + 1]] - tb;
+  const int tf = (wg - row_wave0(A.tile_off, A.fcb, f)) * 4 + rw;
+  if (__ballot(tf < nt_frame) == 0) return;
+  const bool has_tile = tf < nt_frame;
+  const int T = tb + (has_tile ? tf : 0);
+  const int g = A.tile_cell[T];
+  const int o0 = A.off[g];
+  const int cnt = A.off[g + 1] - o0;
+  const int tl = T - A.tile_off[g];
+  const int srel = tl * kTile + li;
+  bool valid = has_tile && srel < cnt;
+  const int s = o0 + (valid ? srel : 0);
+  const bool frame_ok = (A.fflags[f] & kFrameNonFinite) == 0 && d.r_safe >= 1;
+  if (!frame_ok) {
+    if (valid) defer_source(s, A.defer_list, A.defer_count);
+    return;
+  }
+  const int k = d.k;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int cb = static_cast<int>(d.cell_base);
+  const int c = g - cb;
+  const int hx = c % nx;
+  const int hy = (c / nx) % ny;
+  const int hz = c / (nx * ny);
+  const Cand mc = A.cand[s];
+  const double px = mc.x, py = mc.y, pz = mc.z;
+  const int me = static_cast<int>(mc.id);
+  const double inf = as_f64(kInfBits);
+  double sl[3], sh[3];
+  sl[0] = red16_min(valid ? px : inf);
+  sh[0] = red16_max(valid ? px : -inf);
+  sl[1] = red16_min(valid ? py : inf);
+  sh[1] = red16_max(valid ? py : -inf);
+  sl[2] = red16_min(valid ? pz : inf);
+  sh[2] = red16_max(valid ? pz : -inf);
+
+  // lattice shift of a stencil cell from its wrap code (na+1)*9+(nb+1)*3+nc+1
+  auto shift_of = [&](int code, double& x, double& y, double& z) {
+    const int na = code / 9 - 1, nb = (code / 3) % 3 - 1, nc = code % 3 - 1;
+    if (code == 13) {
+      x = y = z = 0.0;
+    } else if (d.ortho) {
+      x = static_cast<double>(na) * d.widths[0];
+      y = static_cast<double>(nb) * d.widths[1];
+      z = static_cast<double>(nc) * d.widths[2];
+    } else {
+      const double fa = na, fb = nb, fc = nc;
+      x = fa * d.h[0] + fb * d.h[3] + fc * d.h[6];
+      y = fa * d.h[1] + fb * d.h[4] + fc * d.h[7];
+      z = fa * d.h[2] + fb * d.h[5] + fc * d.h[8];
+    }
+  };
+
+  Top<KC> h;
+  top_init<KC>(h, k);
+  double min_rej = inf;
+  double thr = inf;
+  bool live = valid;
+  double rthr = red16_max(live ? thr : -inf);  // -inf: no live source
+  int nbuf = 0;
+  int done_reach = 0;
+  long long* bd = &s_bd[wv][0][lane];
+  int* bj = &s_bj[wv][0][lane];
+  int* lslot = &s_slot[wv][rw][0];
+  int* lmeta = &s_meta[wv][rw][0];
+  unsigned st_tests = 0, st_tiles = 0, st_merges = 0, st_rounds = 0,
+           st_cells = 0, st_exact = 0;
+
+  auto flush = [&]() {
+    if (nbuf > 0) {
+      st_merges += 1;
+      insert_merge<KC>(h, bd, bj, nbuf, min_rej);
+      nbuf = 0;
+      thr = as_f64(h.d[KC - 1]);
+    }
+    rthr = red16_max(live ? thr : -inf);
+  };
+
+  double cx = 0.0, cy = 0.0, cz = 0.0;
+  int cj = -1;
+  auto step = [&](auto I) {
+    constexpr int ii = decltype(I)::value;
+    const double qx = row_bcast<ii>(cx);
+    const double qy = row_bcast<ii>(cy);
+    const double qz = row_bcast<ii>(cz);
+    const int qj = row_bcast<ii>(cj);
+    const double ddx = qx - px, ddy = qy - py, ddz = qz - pz;
+    const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+    // empty candidate slots carry NaN coordinates: never accepted
+    if (live && d2 <= thr && qj != me) {
+      bd[nbuf * kWave] = __double_as_longlong(d2);
+      bj[nbuf * kWave] = qj;
+      nbuf += 1;
+    }
+  };
+  auto quad = [&](auto B) {
+    constexpr int b0 = decltype(B)::value;
+    step(std::integral_constant<int, b0>{});
+    step(std::integral_constant<int, b0 + 1>{});
+    step(std::integral_constant<int, b0 + 2>{});
+    step(std::integral_constant<int, b0 + 3>{});
+  };
+  // the 16 candidates held in cx..cj (one per lane of each row) against
+  // every source of the row; one merge site checked after each quad
+  auto sweep = [&]() {
+    st_rounds += 1;
+#pragma unroll 1
+    for (int qd = 0; qd < 4; ++qd) {
+      switch (qd) {
+        case 0: quad(std::integral_constant<int, 0>{}); break;
+        case 1: quad(std::integral_constant<int, 4>{}); break;
+        case 2: quad(std::integral_constant<int, 8>{}); break;
+        default: quad(std::integral_constant<int, 12>{}); break;
+      }
+      if (__ballot(nbuf > BC - 4)) flush();
+    }
+  };
+
+  // opening: the row's own tile (its sources are the candidates, no
+  // shift), then the next tile of the home cell in Morton order
+  const int nt_home = (cnt + kTile - 1) >> kTileShift;
+  const int tl_nb = tl + 1 < nt_home ? tl + 1 : tl - 1;  // -1: none
+  const double nan = __builtin_nan("");
+  cx = valid ? px : nan;
+  cy = valid ? py : nan;
+  cz = valid ? pz : nan;
+  cj = valid ? me : -1;
+  sweep();
+  cx = cy = cz = nan;
+  cj = -1;
+  if (has_tile && tl_nb >= 0 && tl_nb * kTile + li < cnt) {
+    const Cand q = A.cand[o0 + tl_nb * kTile + li];
+    cx = q.x;
+    cy = q.y;
+    cz = q.z;
+    cj = static_cast<int>(q.id);
+  }
+  if (__ballot(tl_nb >= 0 && has_tile)) sweep();
+  if (__ballot(nbuf > 0)) flush();
+
+  int lcount = 0;  // entries in this row's list
+  // drain the list: tile i's candidates are loaded while tile i - 1 is
+  // swept
+  auto drain = [&]() {
+    int i = 0;
+    double nxq = nan, nyq = nan, nzq = nan;
+    int njq = -1, ncode = 13;
+    auto fetch = [&](int e) {
+      nxq = nyq = nzq = nan;
+      njq = -1;
+      ncode = 13;
+      if (e < lcount) {
+        const int sl0 = lslot[e];
+        const int meta = lmeta[e];
+        ncode = meta & 31;
+        if (li < (meta >> 5)) {
+          const Cand q = A.cand[sl0 + li];
+          nxq = q.x;
+          nyq = q.y;
+          nzq = q.z;
+          njq = static_cast<int>(q.id);
+        }
+      }
+    };
+    fetch(0);
+#pragma unroll 1
+    while (__ballot(i < lcount)) {
+      double shx, shy, shz;
+      shift_of(ncode, shx, shy, shz);
+      cx = nxq + shx;
+      cy = nyq + shy;
+      cz = nzq + shz;
+      cj = njq;
+      i += 1;
+      fetch(i);
+      st_tiles += 1;
+      sweep();
+    }
+    lcount = 0;
+  };
+
+#pragma unroll 1
+  for (int r = 1;; ++r) {
+    const int ncr = shell_cells(r);
+    // per-row enumeration state: a chunk of 16 cells, one per lane
+    int cc = 0, p0 = 0, total = 0;
+    int ca0 = 0, cn2 = 0, ct0 = 0, ccode = 13, cnt_c = 0, cpre = 0;
+    bool more = rthr >= 0.0;
+#pragma unroll 1
+    while (true) {
+      // fill the list while every filling row has room for 16 entries
+#pragma unroll 1
+      while (true) {
+        const bool fill = more && lcount <= LC - 16;
+        if (!__ballot(fill)) break;
+        if (fill) {
+          if (p0 >= total) {
+            if (cc >= ncr) {
+              more = false;
+            } else {
+              // headers of 16 cells, one per lane
+              st_cells += 1;
+              const int ci = cc + li;
+              cn2 = 0;
+              ca0 = ct0 = 0;
+              ccode = 13;
+              if (ci < ncr) {
+                int dx, dy, dz;
+                surface_cell(r, ci, dx, dy, dz);
+                const int jx = hx + dx, jy = hy + dy, jz = hz + dz;
+                const int ux = jx < 0 ? jx + nx : (jx >= nx ? jx - nx : jx);
+                const int uy = jy < 0 ? jy + ny : (jy >= ny ? jy - ny : jy);
+                const int uz = jz < 0 ? jz + nz : (jz >= nz ? jz - nz : jz);
+                const int na = jx < 0 ? -1 : (jx >= nx ? 1 : 0);
+                const int nb = jy < 0 ? -1 : (jy >= ny ? 1 : 0);
+                const int nc = jz < 0 ? -1 : (jz >= nz ? 1 : 0);
+                const int gq = cb + (uz * ny + uy) * nx + ux;
+                ca0 = A.off[gq];
+                cn2 = A.off[gq + 1] - ca0;
+                ct0 = A.tile_off[gq];
+                ccode = (na + 1) * 9 + (nb + 1) * 3 + (nc + 1);
+                if (cn2 > kTile) {
+                  double shx, shy, shz;
+                  shift_of(ccode, shx, shy, shz);
+                  if (hull_lb(sl, sh, A.cbox[gq], shx, shy, shz) > rthr) cn2 = 0;
+                }
+              }
+              cnt_c = (cn2 + kTile - 1) >> kTileShift;
+              // inclusive prefix of tile counts over the row
+              cpre = cnt_c;
+#pragma unroll
+              for (int dd = 1; dd < 16; dd <<= 1) {
+                const int y = __shfl_up(cpre, dd, 16);
+                if (li >= dd) cpre += y;
+              }
+              total = __shfl(cpre, rbase + 15);
+              p0 = 0;
+              cc += 16;
+            }
+          }
+          if (more && p0 < total) {
+            // 16 tiles of the chunk, one per lane
+            const int p = p0 + li;
+            const bool pv = p < total;
+            int own = 0;
+#pragma unroll
+            for (int step2 = 8; step2 >= 1; step2 >>= 1) {
+              const int probe = own + step2 - 1;
+              const int pr = __shfl(cpre, rbase + (probe < 16 ? probe : 15));
+              if (probe < 16 && pr <= p) own += step2;
+            }
+            own = own < 16 ? own : 15;
+            const int oa0 = __shfl(ca0, rbase + own);
+            const int on2 = __shfl(cn2, rbase + own);
+            const int ot0 = __shfl(ct0, rbase + own);
+            const int ocode = __shfl(ccode, rbase + own);
+            const int oex = __shfl(cpre - cnt_c, rbase + own);
+            const int t = p - oex;
+            const bool opening = r == 1 && cc == 16 && own == 0 &&
+                                 (t == tl || t == tl_nb);
+            bool need = false;
+            double bx0 = 0, bx1 = 0, by0 = 0, by1 = 0, bz0 = 0, bz1 = 0;
+            if (pv && !opening) {
+              double shx, shy, shz;
+              shift_of(ocode, shx, shy, shz);
+              const Box b = A.tbox[ot0 + t];
+              bx0 = b.lx + shx;
+              bx1 = b.hx + shx;
+              by0 = b.ly + shy;
+              by1 = b.hy + shy;
+              bz0 = b.lz + shz;
+              bz1 = b.hz + shz;
+              st_tests += 1;
+              const double gx = fmax(fmax(bx0 - sh[0], sl[0] - bx1), 0.0);
+              const double gy = fmax(fmax(by0 - sh[1], sl[1] - by1), 0.0);
+              const double gz = fmax(fmax(bz0 - sh[2], sl[2] - bz1), 0.0);
+              need = gx * gx + gy * gy + gz * gz <= rthr;
+            }
+            // exact per-source test of the tiles that passed the row box
+            if (__ballot(need)) {
+              st_exact += 1;
+              const double tq = live ? thr : -inf;
+              bool any = false;
+              auto probe_src = [&](auto I) {
+                constexpr int ii = decltype(I)::value;
+                const double qx = row_bcast<ii>(px);
+                const double qy = row_bcast<ii>(py);
+                const double qz = row_bcast<ii>(pz);
+                const double qt = row_bcast<ii>(tq);
+                const double gx = fmax(fmax(bx0 - qx, qx - bx1), 0.0);
+                const double gy = fmax(fmax(by0 - qy, qy - by1), 0.0);
+                const double gz = fmax(fmax(bz0 - qz, qz - bz1), 0.0);
+                any |= gx * gx + gy * gy + gz * gz <= qt;
+              };
+              probe_src(std::integral_constant<int, 0>{});
+              probe_src(std::integral_constant<int, 1>{});
+              probe_src(std::integral_constant<int, 2>{});
+              probe_src(std::integral_constant<int, 3>{});
+              probe_src(std::integral_constant<int, 4>{});
+              probe_src(std::integral_constant<int, 5>{});
+              probe_src(std::integral_constant<int, 6>{});
+              probe_src(std::integral_constant<int, 7>{});
+              probe_src(std::integral_constant<int, 8>{});
+              probe_src(std::integral_constant<int, 9>{});
+              probe_src(std::integral_constant<int, 10>{});
+              probe_src(std::integral_constant<int, 11>{});
+              probe_src(std::integral_constant<int, 12>{});
+              probe_src(std::integral_constant<int, 13>{});
+              probe_src(std::integral_constant<int, 14>{});
+              probe_src(std::integral_constant<int, 15>{});
+              need = need && any;
+            }
+            const unsigned long long bal = __ballot(need);
+            const unsigned rm = static_cast<unsigned>(bal >> rbase) & 0xffffu;
+            if (need) {
+              const int pos = lcount + __popc(rm & ((1u << li) - 1u));
+              lslot[pos] = oa0 + t * kTile;
+              const int e = on2 - t * kTile;
+              lmeta[pos] = ocode | ((e < kTile ? e : kTile) << 5);
+            }
+            lcount += __popc(rm);
+            p0 += 16;
+          }
+        }
+      }
+      if (__ballot(lcount > 0)) drain();
+      if (!__ballot(more)) break;
+    }
+    if (__ballot(nbuf > 0)) flush();
+    // stop test, src/knearest.rs:466-471
+    if (live && h.j[KC - 1] != kEmptyJ) {
+      const double bound = static_cast<double>(r) * d.cell_min;
+      if (as_f64(h.d[KC - 1]) <= bound * bound) {
+        live = false;
+        done_reach = r;
+      }
+    }
+    // beyond r_safe the stencil wraps onto itself: exact walk
+    if (live && r + 1 > d.r_safe) live = false;
+    rthr = red16_max(live ? thr : -inf);
+    if (!__ballot(live)) break;
+  }
+  if (A.stats != nullptr) {
+    stat_add(A.stats, kStatRowTileTests, st_tests);
+    stat_add(A.stats, kStatRowMerges, st_merges);
+    if (valid) stat_add(A.stats, kStatRowSources, 1);
+    if (li == 0) {
+      stat_add(A.stats, kStatRowTiles, st_tiles);
+      stat_add(A.stats, kStatRowRefines, st_exact);
+    }
+    if (lane == 0) {
+      stat_add(A.stats, kStatRowRounds, st_rounds);
+      stat_add(A.stats, kStatRowCells, st_cells);
+    }
+  }
+  if (!valid) return;
+  const size_t row = static_cast<size_t>(me) * k;
+  if (done_reach == 0 ||
+      !top_emit<KC>(h, k, min_rej, d.out_nn, d.out_d2, row)) {
+    defer_source(s, A.defer_list, A.defer_count);
+    return;
+  }
+  if (d.out_reach) d.out_reach[me] = done_reach;
+}
+
 // Queue every active source of the frames that have no register-heap
 // bucket (k above the largest bucket, or the exact-only mode).
 __global__ __launch_bounds__(256) void k_route_exact(
@@ -1105,6 +1668,55 @@ __global__ __launch_bounds__(256) void k_route_exact(
   if (s >= nact) return;
   const int f = find_frame(fcb, F, scell[s]);
   if (fr[f].kc == 0) defer_source(s, defer_list, defer_count);
+}
+
+// Depth schedule of the lane-per-source walk. Every source of a cell below
+// the dense threshold, in a frame with a register-heap bucket, goes to
+// bucket (heap bucket, clamped predicted depth of its home cell). Pass 1
+// counts, pass 2 places; one atomic per wave and bucket, so each wave's
+// sources stay in cell order inside their bucket. Only the walk order
+// changes: every source still runs the same walk and stop test.
+template <bool kPlace>
+__global__ __launch_bounds__(256) void k_sched(
+    const FrameDesc* __restrict__ fr, const long long* __restrict__ fcb,
+    int F, const int* __restrict__ scell, const int* __restrict__ off,
+    long long ncells, const int* __restrict__ depth, int dense_min,
+    int* __restrict__ count, int* __restrict__ cursor,
+    int* __restrict__ sched) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int nact = off[ncells];
+  int b = -1;
+  if (s < nact) {
+    const int g = scell[s];
+    if (off[g + 1] - off[g] < dense_min) {
+      const int kc = fr[find_frame(fcb, F, g)].kc;
+      if (kc != 0) {
+        int dep = depth[g];
+        dep = dep < 1 ? 1 : (dep > kSchedDepths ? kSchedDepths : dep);
+        b = sched_kc_index(kc) * kSchedDepths + (kSchedDepths - dep);
+      }
+    }
+  }
+  const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll 1
+  for (int q = 0; q < kSchedBuckets; ++q) {
+    const unsigned long long m = __ballot(b == q);
+    if (m == 0) continue;
+    const int leader = __ffsll(static_cast<long long>(m)) - 1;
+    int base = 0;
+    if (lane == leader) {
+      base = atomicAdd(kPlace ? &cursor[q] : &count[q], __popcll(m));
+    }
+    if (kPlace) {
+      base = __shfl(base, leader);
+      if (b == q) {
+        int start = 0;
+        for (int p = 0; p < q; ++p) start += count[p];
+        sched[start + base + __popcll(m & below)] = s;
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------- exact walk
@@ -1285,11 +1897,13 @@ Engine::Engine() {
     if (x >= 1) dense_min_override_ = x;
   }
   // LINKCELL_TPP keeps its meaning of "threads per source": 1 sends every
-  // source to the lane-per-source walk, 64 every cell to the wave-per-cell
+  // source to the lane-per-source walk, 16 selects the row walk (16-lane
+  // rows per 16-source tile), 64 sends every cell to the wave-per-cell
   // walk.
   if (const char* v = std::getenv("LINKCELL_TPP")) {
     const int x = std::atoi(v);
     if (x == 1) dense_min_override_ = kNeverDense;
+    if (x == 16) dense_min_override_ = kRowWalk;
     if (x >= 64) dense_min_override_ = 1;
   }
 }
@@ -1344,7 +1958,8 @@ void Engine::free_all() {
                 &counters_, &off_,      &tile_off_, &item_off_, &items_,
                 &cellslot_, &tmp_key_,  &cand_,
                 &scell_,    &byidx_,  &tbox_,   &cbox_,
-                &defer_,    &heap_d2_,  &heap_idx_, &flags_, &stats_};
+                &defer_,    &heap_d2_,  &heap_idx_, &flags_, &stats_,
+                &sched_,    &sched_cnt_, &tile_cell_};
   for (Buf* b : all) {
     if (b->p != nullptr) (void)hipFree(b->p);
     b->p = nullptr;
@@ -1445,6 +2060,7 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   grow(scell_, sizeof(int) * static_cast<size_t>(ntot));
   grow(byidx_, sizeof(int) * static_cast<size_t>(ntot));
   grow(tbox_, sizeof(Box) * static_cast<size_t>(tcap));
+  grow(tile_cell_, sizeof(int) * static_cast<size_t>(tcap));
   grow(cbox_, sizeof(Box) * static_cast<size_t>(ctot));
   grow(defer_, sizeof(int) * static_cast<size_t>(ntot));
   grow(flags_, sizeof(int) * static_cast<size_t>(F));
@@ -1546,7 +2162,8 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   if (timing) check(hipEventRecord(ev_[2], st), "event record");
 
   k_emit_items<<<blocks_for(ctot, 256), 256, 0, st>>>(
-      as<int>(count_.p), as<int>(item_off_.p), ctot, as<int>(items_.p));
+      as<int>(count_.p), as<int>(item_off_.p), as<int>(tile_off_.p), ctot,
+      as<int>(items_.p), as<int>(tile_cell_.p));
   k_scatter<<<blocks_for(ntot, 256), 256, 0, st>>>(
       as<int4>(cellslot_.p), dfpt, F, ntot, as<int>(off_.p),
       as<long long>(tmp_key_.p));
@@ -1582,6 +2199,7 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   wa.tbox = as<Box>(tbox_.p);
   wa.cbox = as<Box>(cbox_.p);
   wa.dense_min = dense_min_for(frames);
+  wa.tile_cell = as<int>(tile_cell_.p);
   wa.defer_list = as<int>(defer_.p);
   wa.defer_count = counters + 1;
   wa.stats = nullptr;
@@ -1593,9 +2211,39 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
     stats_n_ = nstat;
     wa.stats = as<unsigned long long>(stats_.p);
   }
+  wa.sched = nullptr;
+  wa.sched_count = nullptr;
+  const int* depth = nullptr;
+  if (sched_mode_ == 2) {
+    if (given_depth_ == nullptr || given_depth_n_ != ctot) {
+      throw std::runtime_error(
+          "schedule depth array length must equal the batch's cell count");
+    }
+    depth = given_depth_;
+  }
+  if (depth != nullptr && wa.dense_min != kRowWalk) {
+    grow(sched_, sizeof(int) * static_cast<size_t>(ntot));
+    grow(sched_cnt_, sizeof(int) * 2 * kSchedBuckets);
+    int* cnt = as<int>(sched_cnt_.p);
+    k_zero_u32<<<1, 64, 0, st>>>(as<unsigned>(sched_cnt_.p), 2 * kSchedBuckets);
+    k_sched<false><<<blocks_for(ntot, 256), 256, 0, st>>>(
+        dfr, dfcb, F, as<int>(scell_.p), as<int>(off_.p), ctot, depth,
+        wa.dense_min, cnt, cnt + kSchedBuckets, as<int>(sched_.p));
+    k_sched<true><<<blocks_for(ntot, 256), 256, 0, st>>>(
+        dfr, dfcb, F, as<int>(scell_.p), as<int>(off_.p), ctot, depth,
+        wa.dense_min, cnt, cnt + kSchedBuckets, as<int>(sched_.p));
+    check(hipGetLastError(), "schedule launch");
+    wa.sched = as<int>(sched_.p);
+    wa.sched_count = cnt;
+  }
   const unsigned sparse_blocks = blocks_for(ntot, 256);
+  // row walk: one wave per 4 source tiles, frames on wave boundaries
+  const unsigned row_blocks = blocks_for(tcap / 4 + F + 1, 4);
 #define LC_WALK(KC)                                                           \
-  if (any_fast[KC]) {                                                         \
+  if (any_fast[KC] && wa.dense_min == kRowWalk) {                             \
+    k_walk_rows<KC><<<row_blocks, 256, 0, st>>>(wa);                          \
+    check(hipGetLastError(), "walk launch");                                  \
+  } else if (any_fast[KC]) {                                                  \
     constexpr int W = WalkShape<KC>::kWaves;                                  \
     k_walk_dense<KC><<<static_cast<unsigned>((max_items + W - 1) / W),        \
                        W * kWave, 0, st>>>(wa);                               \
@@ -1656,12 +2304,12 @@ std::string bucket_key(const std::vector<FrameSpec>& frames) {
 }  // namespace
 
 int Engine::dense_min_for(const std::vector<FrameSpec>& frames) const {
-  if (dense_min_override_ > 0) return dense_min_override_;
+  if (dense_min_override_ != 0) return dense_min_override_;
   const std::string key = bucket_key(frames);
   for (const auto& e : tuned_) {
     if (e.first == key) return e.second;
   }
-  return kNeverDense;
+  return kRowWalk;
 }
 
 void Engine::capture(const std::vector<FrameSpec>& frames) {
@@ -1711,7 +2359,7 @@ void Engine::reset_graph() {
 }
 
 int Engine::autotune(const std::vector<FrameSpec>& frames, float* best_ms) {
-  static const int kCands[] = {1, 16, 32, 64, 128, kNeverDense};
+  static const int kCands[] = {kRowWalk, 1, 16, 32, 64, 128, kNeverDense};
   const int keep_override = dense_min_override_;
   const bool keep_timing = timing_;
   int best = kNeverDense;

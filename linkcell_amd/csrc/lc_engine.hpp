@@ -97,9 +97,19 @@ class Engine {
   void set_mode(WalkMode m) { mode_ = m; }
   // Home cells with at least this many sources take the wave-per-cell
   // walk, the rest the lane-per-source walk.
-  // An explicit value overrides the tuned table; 0 returns to it.
-  void set_dense_min(int v) { dense_min_override_ = v < 0 ? 0 : v; }
+  // An explicit value overrides the tuned table; 0 returns to it; -1
+  // selects the row walk (one 16-lane row per 16-source tile).
+  void set_dense_min(int v) { dense_min_override_ = v < -1 ? 0 : v; }
   int dense_min_for(const std::vector<FrameSpec>& frames) const;
+  // Order of the lane-per-source walk: 0 = cell order, 2 = grouped by a
+  // caller-given depth per cell (device array, one int per cell of the
+  // batch in frame order), 1 = by the learned predictor's depths.
+  void set_schedule(int mode) { sched_mode_ = mode; }
+  int schedule() const { return sched_mode_; }
+  void set_given_depth(const int* depth, long long n) {
+    given_depth_ = depth;
+    given_depth_n_ = n;
+  }
   // Device work counters of the walks (diagnostics).
   void set_stats(bool on) { stats_on_ = on; }
   void read_stats(unsigned long long* out, int n);
@@ -116,7 +126,11 @@ class Engine {
   int device_ = -1;
   Buf frames_, fpt_, fcb_, count_, status_, counters_, off_, tile_off_,
       item_off_, items_, cellslot_, tmp_key_, cand_, scell_,
-      byidx_, tbox_, cbox_, defer_, heap_d2_, heap_idx_, flags_, stats_;
+      byidx_, tbox_, cbox_, defer_, heap_d2_, heap_idx_, flags_, stats_,
+      sched_, sched_cnt_, tile_cell_;
+  int sched_mode_ = 0;
+  const int* given_depth_ = nullptr;
+  long long given_depth_n_ = 0;
   void* staging_ = nullptr;
   size_t staging_bytes_ = 0;
   hipEvent_t staging_ev_ = nullptr;

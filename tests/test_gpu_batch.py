@@ -96,7 +96,7 @@ def test_uneven_batch_matches_per_frame_oracle(ws, c4, c4_oracle):
     assert_frames_equal(b.results(), c4_oracle, "batch")
 
 
-@pytest.mark.parametrize("dense_min", [1, 16, 64, 1 << 30])
+@pytest.mark.parametrize("dense_min", [-1, 1, 16, 64, 1 << 30])
 def test_batch_independent_of_walk_split(ws, c4, c4_oracle, dense_min):
     b = DeviceBatch(c4[:10])
     ws.set_dense_min(dense_min)
@@ -170,7 +170,7 @@ def test_graph_launch_without_capture_raises(ws):
 def test_autotune_keeps_results(ws, c4, c4_oracle):
     b = DeviceBatch(c4[:10])
     best, ms = ws.autotune(b.recs)
-    assert best in (1, 16, 32, 64, 128, 1 << 30)
+    assert best in (-1, 1, 16, 32, 64, 128, 1 << 30)
     assert ms > 0.0
     b.clear()
     ws.knearest_batch(b.recs)
