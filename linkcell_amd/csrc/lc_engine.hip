@@ -1262,15 +1262,33 @@ __device__ __forceinline__ int row_wave0(const int* tile_off,
 // reach the row's lanes through DPP broadcasts, while the next tile's loads
 // are in flight. The row's own tile, already in registers, and its Morton
 // neighbour open the walk, so the tests start from a finite k-th distance.
+#ifndef LC_ROW_FILL
+#define LC_ROW_FILL 16
+#endif
+// Resident 256-thread blocks per CU the register budget is cut for: 4
+// waves per SIMD up to KC = 16, 3 for KC = 32 (measured on the clustered
+// and FCC workloads; the walk is bound by memory latency, so occupancy
+// pays more than the few spills it costs).
 template <int KC>
-__global__ __launch_bounds__(256) void k_walk_rows(WalkArgs A) {
+struct RowOcc {
+  static constexpr int kMinBlocks = KC >= 32 ? 3 : 4;
+};
+#ifdef LC_ROW_MINB
+#define LC_ROW_BLOCKS(KC) LC_ROW_MINB
+#else
+#define LC_ROW_BLOCKS(KC) RowOcc<KC>::kMinBlocks
+#endif
+template <int KC, bool kStats>
+__global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A) {
   constexpr int W = 4;   // waves per block
   constexpr int BC = 8;  // buffered candidates per lane between merges
   constexpr int LC = 48; // tile list entries per row
+  constexpr int kFill = LC_ROW_FILL;  // list length at which a drain starts
   __shared__ long long s_bd[W][BC][kWave];
   __shared__ int s_bj[W][BC][kWave];
   __shared__ int s_slot[W][4][LC];
   __shared__ int s_meta[W][4][LC];
+  __shared__ double s_hull[W][4][6];
   const int wv = uni(static_cast<int>(threadIdx.x) / kWave);
   const int lane = threadIdx.x & (kWave - 1);
   const int li = lane & 15;
@@ -1319,13 +1337,26 @@ __global__ __launch_bounds__(256) void k_walk_rows(WalkArgs A) {
   const double px = mc.x, py = mc.y, pz = mc.z;
   const int me = static_cast<int>(mc.id);
   const double inf = as_f64(kInfBits);
-  double sl[3], sh[3];
-  sl[0] = red16_min(valid ? px : inf);
-  sh[0] = red16_max(valid ? px : -inf);
-  sl[1] = red16_min(valid ? py : inf);
-  sh[1] = red16_max(valid ? py : -inf);
-  sl[2] = red16_min(valid ? pz : inf);
-  sh[2] = red16_max(valid ? pz : -inf);
+  // bounding box of the row's sources, kept in LDS (read only while
+  // listing tiles)
+  {
+    const double v0 = red16_min(valid ? px : inf);
+    const double v1 = red16_max(valid ? px : -inf);
+    const double v2 = red16_min(valid ? py : inf);
+    const double v3 = red16_max(valid ? py : -inf);
+    const double v4 = red16_min(valid ? pz : inf);
+    const double v5 = red16_max(valid ? pz : -inf);
+    if (li == 0) {
+      double* hb = s_hull[wv][rw];
+      hb[0] = v0;
+      hb[1] = v1;
+      hb[2] = v2;
+      hb[3] = v3;
+      hb[4] = v4;
+      hb[5] = v5;
+    }
+  }
+  const double* hull = s_hull[wv][rw];
 
   // lattice shift of a stencil cell from its wrap code (na+1)*9+(nb+1)*3+nc+1
   auto shift_of = [&](int code, double& x, double& y, double& z) {
@@ -1361,7 +1392,7 @@ __global__ __launch_bounds__(256) void k_walk_rows(WalkArgs A) {
 
   auto flush = [&]() {
     if (nbuf > 0) {
-      st_merges += 1;
+      if (kStats) st_merges += 1;
       insert_merge<KC>(h, bd, bj, nbuf, min_rej);
       nbuf = 0;
       thr = as_f64(h.d[KC - 1]);
@@ -1396,7 +1427,7 @@ __global__ __launch_bounds__(256) void k_walk_rows(WalkArgs A) {
   // the 16 candidates held in cx..cj (one per lane of each row) against
   // every source of the row; one merge site checked after each quad
   auto sweep = [&]() {
-    st_rounds += 1;
+    if (kStats) st_rounds += 1;
 #pragma unroll 1
     for (int qd = 0; qd < 4; ++qd) {
       switch (qd) {
@@ -1466,7 +1497,7 @@ __global__ __launch_bounds__(256) void k_walk_rows(WalkArgs A) {
       cj = njq;
       i += 1;
       fetch(i);
-      st_tiles += 1;
+      if (kStats) st_tiles += 1;
       sweep();
     }
     lcount = 0;
@@ -1484,7 +1515,7 @@ __global__ __launch_bounds__(256) void k_walk_rows(WalkArgs A) {
       // fill the list while every filling row has room for 16 entries
 #pragma unroll 1
       while (true) {
-        const bool fill = more && lcount <= LC - 16;
+        const bool fill = more && lcount < kFill;
         if (!__ballot(fill)) break;
         if (fill) {
           if (p0 >= total) {
@@ -1492,7 +1523,7 @@ __global__ __launch_bounds__(256) void k_walk_rows(WalkArgs A) {
               more = false;
             } else {
               // headers of 16 cells, one per lane
-              st_cells += 1;
+              if (kStats) st_cells += 1;
               const int ci = cc + li;
               cn2 = 0;
               ca0 = ct0 = 0;
@@ -1515,6 +1546,8 @@ __global__ __launch_bounds__(256) void k_walk_rows(WalkArgs A) {
                 if (cn2 > kTile) {
                   double shx, shy, shz;
                   shift_of(ccode, shx, shy, shz);
+                  const double sl[3] = {hull[0], hull[2], hull[4]};
+                  const double sh[3] = {hull[1], hull[3], hull[5]};
                   if (hull_lb(sl, sh, A.cbox[gq], shx, shy, shz) > rthr) cn2 = 0;
                 }
               }
@@ -1563,15 +1596,15 @@ __global__ __launch_bounds__(256) void k_walk_rows(WalkArgs A) {
               by1 = b.hy + shy;
               bz0 = b.lz + shz;
               bz1 = b.hz + shz;
-              st_tests += 1;
-              const double gx = fmax(fmax(bx0 - sh[0], sl[0] - bx1), 0.0);
-              const double gy = fmax(fmax(by0 - sh[1], sl[1] - by1), 0.0);
-              const double gz = fmax(fmax(bz0 - sh[2], sl[2] - bz1), 0.0);
+              if (kStats) st_tests += 1;
+              const double gx = fmax(fmax(bx0 - hull[1], hull[0] - bx1), 0.0);
+              const double gy = fmax(fmax(by0 - hull[3], hull[2] - by1), 0.0);
+              const double gz = fmax(fmax(bz0 - hull[5], hull[4] - bz1), 0.0);
               need = gx * gx + gy * gy + gz * gz <= rthr;
             }
             // exact per-source test of the tiles that passed the row box
             if (__ballot(need)) {
-              st_exact += 1;
+              if (kStats) st_exact += 1;
               const double tq = live ? thr : -inf;
               bool any = false;
               auto probe_src = [&](auto I) {
@@ -1633,7 +1666,7 @@ __global__ __launch_bounds__(256) void k_walk_rows(WalkArgs A) {
     rthr = red16_max(live ? thr : -inf);
     if (!__ballot(live)) break;
   }
-  if (A.stats != nullptr) {
+  if (kStats) {
     stat_add(A.stats, kStatRowTileTests, st_tests);
     stat_add(A.stats, kStatRowMerges, st_merges);
     if (valid) stat_add(A.stats, kStatRowSources, 1);
@@ -2241,7 +2274,11 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   const unsigned row_blocks = blocks_for(tcap / 4 + F + 1, 4);
 #define LC_WALK(KC)                                                           \
   if (any_fast[KC] && wa.dense_min == kRowWalk) {                             \
-    k_walk_rows<KC><<<row_blocks, 256, 0, st>>>(wa);                          \
+    if (wa.stats != nullptr) {                                                \
+      k_walk_rows<KC, true><<<row_blocks, 256, 0, st>>>(wa);                  \
+    } else {                                                                  \
+      k_walk_rows<KC, false><<<row_blocks, 256, 0, st>>>(wa);                 \
+    }                                                                         \
     check(hipGetLastError(), "walk launch");                                  \
   } else if (any_fast[KC]) {                                                  \
     constexpr int W = WalkShape<KC>::kWaves;                                  \
