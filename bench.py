@@ -6,6 +6,9 @@ one full search (fold, count, scan, scatter, cell sort, walk, packed
 outputs) through the lc_gpu_* C waist. Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3]
+
+--config c4 runs the uneven 256-frame batch through the captured-graph
+path instead (its own JSON line).
 """
 from __future__ import annotations
 
@@ -121,6 +124,91 @@ def cpu_baseline(xyz, cell, k, hint, mask, target_s=12.0):
                        f"sources ({walk:.2f} s), extrapolated to {n} sources")}
 
 
+PMC_PROFILE = ROOT / "profiles" / "walk_rows_clustered_4M_k16_pmc.txt"
+
+
+def pmc_traffic(workload_name):
+    """HBM bytes per walk launch from the committed rocprofv3 --pmc summary
+    of the same workload (FETCH_SIZE + WRITE_SIZE of the walk kernel, each
+    scaled by the copy-kernel calibration recorded there), or None."""
+    if not PMC_PROFILE.exists():
+        return None, None
+    vals = {}
+    for line in PMC_PROFILE.read_text().splitlines():
+        if line.startswith("#") or ":" not in line:
+            continue
+        key, _, val = line.partition(":")
+        vals[key.strip()] = val.strip()
+    if vals.get("workload") != workload_name or "traffic_bytes_per_launch" not in vals:
+        return None, None
+    return float(vals["traffic_bytes_per_launch"]), PMC_PROFILE.name
+
+
+def copy_calibration(mib):
+    """One elementwise kernel of known traffic (reads and writes mib MiB),
+    for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = (mib << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    log(f"bench: copy calibration kernel, {mib} MiB read + {mib} MiB written")
+
+
+def bench_batch(args):
+    """Uneven batch of 256 frames (n from 1e3 to 2e5, orthorhombic and
+    triclinic, uniform and clustered, k in {4, 12, 32}) in one pipeline,
+    captured once into a hipGraph and replayed."""
+    import torch
+    import linkcell_amd as lc
+    from linkcell_amd import workloads as wl
+    frames = wl.batched_c4(256)
+    dev = torch.device("cuda:0")
+    keep, recs = [], []
+    ntot = 0
+    for fr in frames:
+        n, k = len(fr["xyz"]), fr["k"]
+        x = torch.from_numpy(np.ascontiguousarray(fr["xyz"])).to(dev)
+        nn = torch.empty((n, k), dtype=torch.int32, device=dev)
+        d2 = torch.empty((n, k), dtype=torch.float64, device=dev)
+        keep += [x, nn, d2]
+        c = np.asarray(fr["cell"], dtype=np.float64)
+        recs.append(dict(xyz=x.data_ptr(), n=n, cell=list(c.reshape(9)) + [0.0, 0.0, 0.0],
+                         k=k, cell_hint=fr["hint"], out_nn=nn.data_ptr(),
+                         out_d2=d2.data_ptr()))
+        ntot += n
+    ws = lc.Workspace()
+    torch.cuda.synchronize()
+    best, tune_ms = ws.autotune(recs)
+    log(f"bench: batch of {len(frames)} frames, {ntot} points; tuned split {best} "
+        f"({tune_ms:.3f} ms)")
+    ws.graph_capture(recs)
+    for _ in range(args.warmup):
+        ws.graph_launch(wait=False)
+    ws.wait()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ws.graph_launch(wait=False)
+    ws.wait()
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    out = {
+        "metric": "source-queries/sec, 256 uneven frames in one captured pipeline",
+        "value": ntot / (ms_step * 1e-3),
+        "unit": "source-queries/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded generator, see linkcell_amd/workloads.py)",
+        "config": {"workload": "uneven-batch-256-frames", "points": ntot,
+                   "k": [4, 12, 32], "split": best, "graph": True},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,15 +218,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", type=int, default=0)
     ap.add_argument("--dense-min", type=int, default=None,
-                    help="home-cell occupancy at which the wave-per-cell walk takes over")
+                    help="walk split: -1 row walk, else the home-cell occupancy at "
+                         "which the wave-per-cell walk takes over")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MiB",
+                    help="also run one elementwise copy kernel of this size "
+                         "(for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding is a "
               "design only (see DESIGN.md), so --gpus must be 1", file=sys.stderr)
         raise SystemExit(2)
 
-    if args.dense_min is not None:
-        os.environ["LINKCELL_DENSE_MIN"] = str(args.dense_min)
+    if args.config == "c4":
+        return bench_batch(args)
     import torch
     import linkcell_amd as lc
 
@@ -159,6 +251,8 @@ def main():
         c12 = c12 + [0.0, 0.0, 0.0]
     ws = lc.Workspace()
     ws.set_mode(args.mode)
+    if args.dense_min is not None:
+        ws.set_dense_min(args.dense_min)
     mptr = dm.data_ptr() if dm is not None else 0
     torch.cuda.synchronize()
 
@@ -197,6 +291,16 @@ def main():
     cid, plan = fold_cells(xyz, cell, hint)
     bwalk, n_act_b = walk_bytes(cid, plan, reach, k)
     t_walk = (kern["walk"] + kern["exact"]) * 1e-3
+    split = args.dense_min if args.dense_min is not None else -1
+    kc = 4 if k <= 4 else 8 if k <= 8 else 16 if k <= 16 else 32 if k <= 32 else 0
+    if args.mode == 1 or kc == 0:
+        walk_kernel = "k_walk_exact"
+    elif split == -1:
+        walk_kernel = f"k_walk_rows<{kc}> (+ k_walk_exact for deferred sources)"
+    else:
+        walk_kernel = (f"k_walk_dense<{kc}> + k_walk_sparse<{kc}> "
+                       "(+ k_walk_exact for deferred sources)")
+    traffic, traffic_src = pmc_traffic(cfg["name"]) if split == -1 and args.mode == 0 else (None, None)
     achieved = bwalk / t_walk / 1e9
     value = n_act / (ms_step * 1e-3)
     out = {
@@ -222,9 +326,13 @@ def main():
         "reach_hist": np.bincount(reach).tolist(),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": bwalk,
-                     "kernel": "k_walk_fast (+ k_walk_exact for deferred sources)"},
+                     "traffic": traffic, "bytes_per_launch": bwalk,
+                     "kernel": walk_kernel},
     }
+    if traffic_src:
+        out["roofline"]["traffic_source"] = traffic_src
+    if args.copy_calib:
+        copy_calibration(args.copy_calib)
     if not args.no_cpu:
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(xyz, cell, k, hint, mask)
