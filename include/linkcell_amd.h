@@ -81,27 +81,29 @@ int lc_gpu_autotune(lc_gpu_workspace *ws, const lc_gpu_frame *frames,
 #define LC_GPU_ROW_WALK (-1)
 int lc_gpu_set_dense_min(lc_gpu_workspace *ws, int dense_min);
 
-/* Order in which the lane-per-source walk takes its sources. Only the
- * order changes: each source runs the same walk and stop test, so results
- * are identical under every schedule.
- *   LC_GPU_SCHED_CELL     cell order (the default)
- *   LC_GPU_SCHED_LEARNED  grouped by the depth the workspace's trained
- *                         predictor gives each home cell (cell order until
- *                         one is trained or loaded)
- *   LC_GPU_SCHED_GIVEN    grouped by the per-cell depths set with
- *                         lc_gpu_sched_set_depth
- * Groups run deepest first, so that long waves start early. */
+/* Walk schedule. Results are identical under every schedule.
+ *   LC_GPU_SCHED_CELL     plain walk (the default)
+ *   LC_GPU_SCHED_LEARNED  the workspace's trained predictor gives each home
+ *                         cell a k-th-neighbour distance (plain walk until a
+ *                         predictor is trained or loaded)
+ *   LC_GPU_SCHED_GIVEN    the per-cell distances set with
+ *                         lc_gpu_sched_set_prediction
+ * The distance caps the walk's pruning threshold from the first tile on; at
+ * every shell end a source whose heap is full is exact under the cap, and
+ * one that is not is finished by the exact-order walk when the cap could
+ * have hidden the reference's stop. The lane-per-source walk also groups
+ * its sources by the predicted depth (ceil(distance / cell_min)). */
 #define LC_GPU_SCHED_CELL 0
 #define LC_GPU_SCHED_LEARNED 1
 #define LC_GPU_SCHED_GIVEN 2
 int lc_gpu_set_schedule(lc_gpu_workspace *ws, int mode);
 
-/* Depth per home cell for LC_GPU_SCHED_GIVEN: a device array of n_cells
- * ints, one per cell of the batch (frames in order, cells in the
- * reference's (cz*ny + cy)*nx + cx numbering). Kept by pointer; it must
- * stay valid while searches use it. */
-int lc_gpu_sched_set_depth(lc_gpu_workspace *ws, const int *depth,
-                           size_t n_cells);
+/* Predicted k-th-neighbour distance per home cell for LC_GPU_SCHED_GIVEN:
+ * a device array of n_cells floats, one per cell of the batch (frames in
+ * order, cells in the reference's (cz*ny + cy)*nx + cx numbering). Kept by
+ * pointer; it must stay valid while searches use it. */
+int lc_gpu_sched_set_prediction(lc_gpu_workspace *ws, const float *kth_dist,
+                                size_t n_cells);
 
 /* Device work counters of the walk kernels (diagnostics). When on, each
  * search accumulates them; lc_gpu_last_stats waits for the last search

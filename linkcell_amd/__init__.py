@@ -145,10 +145,11 @@ class Workspace:
         """Order of the lane-per-source walk (results never change)."""
         self._check(self._lib.lc_gpu_set_schedule(self._ws, int(mode)))
 
-    def sched_set_depth(self, depth_ptr: int, n_cells: int):
-        """Per-cell depths (device int32 array) for SCHED_GIVEN."""
-        self._check(self._lib.lc_gpu_sched_set_depth(
-            self._ws, C.c_void_p(depth_ptr or None), int(n_cells)))
+    def sched_set_prediction(self, kth_ptr: int, n_cells: int):
+        """Per-cell k-th-neighbour distances (device float32 array) for
+        SCHED_GIVEN."""
+        self._check(self._lib.lc_gpu_sched_set_prediction(
+            self._ws, C.c_void_p(kth_ptr or None), int(n_cells)))
 
     def set_stats(self, on: bool):
         self._check(self._lib.lc_gpu_set_stats(self._ws, 1 if on else 0))

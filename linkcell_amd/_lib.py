@@ -25,7 +25,7 @@ EXPORTS = (
     "lc_gpu_last_timing", "lc_gpu_set_dense_min", "lc_gpu_set_stats",
     "lc_gpu_last_stats", "lc_gpu_knearest_batch", "lc_gpu_graph_capture",
     "lc_gpu_graph_launch", "lc_gpu_graph_reset", "lc_gpu_autotune",
-    "lc_gpu_set_schedule", "lc_gpu_sched_set_depth",
+    "lc_gpu_set_schedule", "lc_gpu_sched_set_prediction",
 )
 
 
@@ -93,7 +93,7 @@ _PROTOS = {
     "lc_gpu_graph_reset": (C.c_int, [_vp]),
     "lc_gpu_autotune": (C.c_int, [_vp, C.POINTER(lc_gpu_frame), _sz, _ip, _dp]),
     "lc_gpu_set_schedule": (C.c_int, [_vp, C.c_int]),
-    "lc_gpu_sched_set_depth": (C.c_int, [_vp, _vp, _sz]),
+    "lc_gpu_sched_set_prediction": (C.c_int, [_vp, _vp, _sz]),
 }
 
 _lib = None

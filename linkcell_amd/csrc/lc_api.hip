@@ -508,13 +508,14 @@ int lc_gpu_set_schedule(lc_gpu_workspace* ws, int mode) {
   return 0;
 }
 
-int lc_gpu_sched_set_depth(lc_gpu_workspace* ws, const int* depth,
-                           size_t n_cells) {
-  if (ws == nullptr || (depth == nullptr && n_cells != 0)) {
+int lc_gpu_sched_set_prediction(lc_gpu_workspace* ws, const float* kth_dist,
+                                size_t n_cells) {
+  if (ws == nullptr || (kth_dist == nullptr && n_cells != 0)) {
     set_gpu_err("null pointer");
     return 1;
   }
-  ws->ws.impl()->engine.set_given_depth(depth, static_cast<long long>(n_cells));
+  ws->ws.impl()->engine.set_given_prediction(kth_dist,
+                                             static_cast<long long>(n_cells));
   set_gpu_err(nullptr);
   return 0;
 }

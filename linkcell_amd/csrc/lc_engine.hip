@@ -679,6 +679,9 @@ struct WalkArgs {
   const int* sched;
   const int* sched_count;  // kSchedBuckets run lengths
   const int* tile_cell;    // home cell of every 16-slot source tile
+  // optional predicted k-th-neighbour distance per home cell (null = plain
+  // walk): the row walk's starting threshold
+  const float* pred;
 };
 
 enum {
@@ -1378,7 +1381,15 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   Top<KC> h;
   top_init<KC>(h, k);
   double min_rej = inf;
-  double thr = inf;
+  // threshold cap from the predicted k-th distance of the home cell (+inf
+  // without a prediction). Candidates beyond it are never kept; the shell-
+  // end check below proves the kept set exact or defers the source.
+  double thr_cap = inf;
+  if (A.pred != nullptr) {
+    const double q = static_cast<double>(A.pred[g]);
+    thr_cap = q * q;
+  }
+  double thr = thr_cap;
   bool live = valid;
   double rthr = red16_max(live ? thr : -inf);  // -inf: no live source
   int nbuf = 0;
@@ -1395,7 +1406,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       if (kStats) st_merges += 1;
       insert_merge<KC>(h, bd, bj, nbuf, min_rej);
       nbuf = 0;
-      thr = as_f64(h.d[KC - 1]);
+      thr = fmin(as_f64(h.d[KC - 1]), thr_cap);
     }
     rthr = red16_max(live ? thr : -inf);
   };
@@ -1654,12 +1665,19 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     }
     if (__ballot(nbuf > 0)) flush();
     // stop test, src/knearest.rs:466-471
+    const double bound = static_cast<double>(r) * d.cell_min;
     if (live && h.j[KC - 1] != kEmptyJ) {
-      const double bound = static_cast<double>(r) * d.cell_min;
+      // a full heap holds only d2 <= thr_cap, and everything skipped or
+      // rejected lies beyond it: the heap is the exact top-k of the shells
+      // walked so far
       if (as_f64(h.d[KC - 1]) <= bound * bound) {
         live = false;
         done_reach = r;
       }
+    } else if (live && thr_cap < bound * bound) {
+      // not full: the plain walk might have filled its heap with candidates
+      // beyond the cap and stopped here; only the exact walk can tell
+      live = false;
     }
     // beyond r_safe the stencil wraps onto itself: exact walk
     if (live && r + 1 > d.r_safe) live = false;
@@ -1713,7 +1731,7 @@ template <bool kPlace>
 __global__ __launch_bounds__(256) void k_sched(
     const FrameDesc* __restrict__ fr, const long long* __restrict__ fcb,
     int F, const int* __restrict__ scell, const int* __restrict__ off,
-    long long ncells, const int* __restrict__ depth, int dense_min,
+    long long ncells, const float* __restrict__ pred, int dense_min,
     int* __restrict__ count, int* __restrict__ cursor,
     int* __restrict__ sched) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1723,9 +1741,13 @@ __global__ __launch_bounds__(256) void k_sched(
   if (s < nact) {
     const int g = scell[s];
     if (off[g + 1] - off[g] < dense_min) {
-      const int kc = fr[find_frame(fcb, F, g)].kc;
+      const FrameDesc& d = fr[find_frame(fcb, F, g)];
+      const int kc = d.kc;
       if (kc != 0) {
-        int dep = depth[g];
+        // predicted depth: the first shell whose stop bound covers the
+        // predicted k-th distance
+        const double q = static_cast<double>(pred[g]) / d.cell_min;
+        int dep = q < kSchedDepths ? static_cast<int>(ceil(q)) : kSchedDepths;
         dep = dep < 1 ? 1 : (dep > kSchedDepths ? kSchedDepths : dep);
         b = sched_kc_index(kc) * kSchedDepths + (kSchedDepths - dep);
       }
@@ -2246,14 +2268,17 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   }
   wa.sched = nullptr;
   wa.sched_count = nullptr;
-  const int* depth = nullptr;
+  wa.pred = nullptr;
+  const float* pred = nullptr;
   if (sched_mode_ == 2) {
-    if (given_depth_ == nullptr || given_depth_n_ != ctot) {
+    if (given_pred_ == nullptr || given_pred_n_ != ctot) {
       throw std::runtime_error(
-          "schedule depth array length must equal the batch's cell count");
+          "schedule prediction length must equal the batch's cell count");
     }
-    depth = given_depth_;
+    pred = given_pred_;
   }
+  wa.pred = pred;
+  const float* depth = pred;
   if (depth != nullptr && wa.dense_min != kRowWalk) {
     grow(sched_, sizeof(int) * static_cast<size_t>(ntot));
     grow(sched_cnt_, sizeof(int) * 2 * kSchedBuckets);

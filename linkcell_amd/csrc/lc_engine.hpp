@@ -101,14 +101,17 @@ class Engine {
   // selects the row walk (one 16-lane row per 16-source tile).
   void set_dense_min(int v) { dense_min_override_ = v < -1 ? 0 : v; }
   int dense_min_for(const std::vector<FrameSpec>& frames) const;
-  // Order of the lane-per-source walk: 0 = cell order, 2 = grouped by a
-  // caller-given depth per cell (device array, one int per cell of the
-  // batch in frame order), 1 = by the learned predictor's depths.
+  // Walk schedule: 0 = plain; 1 = the learned predictor's k-th-neighbour
+  // distance per home cell; 2 = a caller-given distance per cell (device
+  // array of floats, one per cell of the batch in frame order). The
+  // distance seeds the walk's pruning threshold (checked exactly at every
+  // shell end; a source it does not cover is finished by the exact walk)
+  // and orders the lane-per-source walk by predicted depth.
   void set_schedule(int mode) { sched_mode_ = mode; }
   int schedule() const { return sched_mode_; }
-  void set_given_depth(const int* depth, long long n) {
-    given_depth_ = depth;
-    given_depth_n_ = n;
+  void set_given_prediction(const float* kth, long long n) {
+    given_pred_ = kth;
+    given_pred_n_ = n;
   }
   // Device work counters of the walks (diagnostics).
   void set_stats(bool on) { stats_on_ = on; }
@@ -129,8 +132,8 @@ class Engine {
       byidx_, tbox_, cbox_, defer_, heap_d2_, heap_idx_, flags_, stats_,
       sched_, sched_cnt_, tile_cell_;
   int sched_mode_ = 0;
-  const int* given_depth_ = nullptr;
-  long long given_depth_n_ = 0;
+  const float* given_pred_ = nullptr;
+  long long given_pred_n_ = 0;
   void* staging_ = nullptr;
   size_t staging_bytes_ = 0;
   hipEvent_t staging_ev_ = nullptr;
