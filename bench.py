@@ -5,6 +5,12 @@ FP64 points and k = 16 (hint 3.0), inputs resident in HBM. One step is
 one full search (fold, count, scan, scatter, cell sort, walk, packed
 outputs) through the lc_gpu_* C waist. Prints one JSON line.
 
+The plain walk is timed first; then the learned scheduler is trained on 32
+further frames of the same mixture (labels from the plain walk, training on
+the device), and the learned walk is timed, checked bitwise against the
+plain walk's outputs. `value` is the learned walk's throughput; the plain
+walk's sits under "plain" (--schedule plain skips the learned leg).
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3]
 
 --config c4 runs the uneven 256-frame batch through the captured-graph
@@ -124,24 +130,27 @@ def cpu_baseline(xyz, cell, k, hint, mask, target_s=12.0):
                        f"sources ({walk:.2f} s), extrapolated to {n} sources")}
 
 
-PMC_PROFILE = ROOT / "profiles" / "walk_rows_clustered_4M_k16_pmc.txt"
+PMC_PROFILES = {"plain": ROOT / "profiles" / "walk_rows_clustered_4M_k16_pmc.txt",
+                "learned": ROOT / "profiles" / "walk_rows_learned_clustered_4M_k16_pmc.txt"}
 
 
-def pmc_traffic(workload_name):
-    """HBM bytes per walk launch from the committed rocprofv3 --pmc summary
-    of the same workload (FETCH_SIZE + WRITE_SIZE of the walk kernel, each
-    scaled by the copy-kernel calibration recorded there), or None."""
-    if not PMC_PROFILE.exists():
+def pmc_traffic(workload_name, schedule):
+    """HBM bytes per walk of the given schedule from the committed rocprofv3
+    --pmc summary of the same workload (FETCH_SIZE + WRITE_SIZE of the walk
+    kernels, each scaled by the copy-kernel calibration recorded there), or
+    None."""
+    prof = PMC_PROFILES.get(schedule)
+    if prof is None or not prof.exists():
         return None, None
     vals = {}
-    for line in PMC_PROFILE.read_text().splitlines():
+    for line in prof.read_text().splitlines():
         if line.startswith("#") or ":" not in line:
             continue
         key, _, val = line.partition(":")
         vals[key.strip()] = val.strip()
     if vals.get("workload") != workload_name or "traffic_bytes_per_launch" not in vals:
         return None, None
-    return float(vals["traffic_bytes_per_launch"]), PMC_PROFILE.name
+    return float(vals["traffic_bytes_per_launch"]), prof.name
 
 
 def copy_calibration(mib):
@@ -223,6 +232,13 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MiB",
                     help="also run one elementwise copy kernel of this size "
                          "(for scaling HBM counters)")
+    ap.add_argument("--schedule", choices=("learned", "plain"), default="learned",
+                    help="headline schedule (learned: train the scheduler on "
+                         "--train-frames frames of the same mixture, then time it "
+                         "beside the plain walk)")
+    ap.add_argument("--train-frames", type=int, default=32)
+    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--quantile", type=float, default=0.98)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding is a "
@@ -260,48 +276,120 @@ def main():
         ws.knearest_ptr(dx.data_ptr(), n, 1, c12, k, nn.data_ptr(), d2.data_ptr(),
                         mptr, hint, rr.data_ptr() if reach else 0, wait)
 
+    def leg(tag):
+        """warmup, K timed steps, per-kernel hipEvent timings, work counters"""
+        for _ in range(args.warmup):
+            step()
+        ws.wait()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        ws.wait()
+        torch.cuda.synchronize()
+        ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+        log(f"bench: {tag}: timed {args.steps} steps, {ms_step:.3f} ms/step")
+        ws.set_timing(True)
+        kt = []
+        for _ in range(max(3, min(args.steps, 10))):
+            step(wait=True)
+            kt.append(ws.last_timing())
+        ws.set_timing(False)
+        kern = {key: float(np.mean([t[key] for t in kt])) for key in kt[0]}
+        ws.set_stats(True)
+        step(wait=True)
+        work = ws.last_stats()
+        ws.set_stats(False)
+        return ms_step, kern, work
+
+    # plain schedule (also records every source's final reach for B_walk)
+    ws.set_schedule(0)
     step(wait=True, reach=True)
     reach = rr.cpu().numpy()
-    for _ in range(args.warmup):
-        step()
-    ws.wait()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    ws.wait()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    log(f"bench: timed {args.steps} steps, {ms_step:.3f} ms/step")
-
-    # per-kernel hipEvent timings on the workspace stream
-    ws.set_timing(True)
-    kt = []
-    for _ in range(max(3, min(args.steps, 10))):
-        step(wait=True)
-        kt.append(ws.last_timing())
-    ws.set_timing(False)
-    kern = {key: float(np.mean([t[key] for t in kt])) for key in kt[0]}
-    ws.set_stats(True)
-    step(wait=True)
-    work = ws.last_stats()
-    ws.set_stats(False)
+    ms_plain, kern_plain, work_plain = leg("plain")
     n_act = int((reach > 0).sum()) if mask is None else int(mask.sum())
     cid, plan = fold_cells(xyz, cell, hint)
-    bwalk, n_act_b = walk_bytes(cid, plan, reach, k)
-    t_walk = (kern["walk"] + kern["exact"]) * 1e-3
+    bwalk, _ = walk_bytes(cid, plan, reach, k)
     split = args.dense_min if args.dense_min is not None else -1
     kc = 4 if k <= 4 else 8 if k <= 8 else 16 if k <= 16 else 32 if k <= 32 else 0
-    if args.mode == 1 or kc == 0:
-        walk_kernel = "k_walk_exact"
-    elif split == -1:
-        walk_kernel = f"k_walk_rows<{kc}> (+ k_walk_exact for deferred sources)"
+
+    def roof(kern):
+        t_walk = (kern["walk"] + kern["exact"]) * 1e-3
+        achieved = bwalk / t_walk / 1e9
+        return t_walk, achieved
+
+    t_walk_plain, ach_plain = roof(kern_plain)
+    plain = {"value": n_act / (ms_plain * 1e-3), "ms_per_step": ms_plain,
+             "walk_ms": t_walk_plain * 1e3, "walk_qps": n_act / t_walk_plain,
+             "roofline_frac": ach_plain / HBM_PEAK_GBS, "kernels_ms": kern_plain,
+             "walk_work": work_plain}
+
+    learned = None
+    learned_ok = (args.schedule == "learned" and args.config == "c3" and kc != 0
+                  and split == -1 and args.mode == 0)
+    if learned_ok:
+        ref_nn, ref_d2 = nn.clone(), d2.clone()
+        tn = torch.empty((n, k), dtype=torch.int32, device=dev)
+        td = torch.empty((n, k), dtype=torch.float64, device=dev)
+        t0 = time.perf_counter()
+        ws.sched_clear()
+        for f in range(1, args.train_frames + 1):
+            tx = wl_device_frame(f, dev)
+            # drawn on torch's stream; the workspace searches on its own
+            torch.cuda.synchronize()
+            ws.knearest_ptr(tx.data_ptr(), n, 1, c12, k, tn.data_ptr(), td.data_ptr(),
+                            0, hint, 0, True)
+            added = ws.sched_collect()
+            del tx
+            if f % 8 == 0 or f == 1:
+                log(f"bench: training frame {f}: {added} tiles, "
+                    f"{time.perf_counter() - t0:.2f} s")
+        samples = ws.sched_samples()
+        t1 = time.perf_counter()
+        loss, margin = ws.sched_train(epochs=args.epochs, batch=4096, lr=2e-3, seed=1,
+                                      quantile=args.quantile)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        del tn, td
+        log(f"bench: learned scheduler: {args.train_frames} frames, {samples} tiles "
+            f"collected in {t1 - t0:.2f} s, trained in {t2 - t1:.2f} s "
+            f"(loss {loss:.4f}, margin {margin:.3f})")
+        ws.set_schedule(1)
+        step(wait=True)
+        same = bool(torch.equal(nn, ref_nn)) and bool(
+            torch.equal(d2.view(torch.int64), ref_d2.view(torch.int64)))
+        if not same:
+            raise SystemExit("bench: the learned walk differs from the plain walk")
+        ms_l, kern_l, work_l = leg("learned")
+        ws.set_timing(True)
+        step(wait=True)
+        retried = ws.last_retried()
+        ws.set_timing(False)
+        ntiles = ws.tile_count()
+        learned = {"ms_per_step": ms_l, "kernels_ms": kern_l, "walk_work": work_l,
+                   "train_frames": args.train_frames, "samples": samples,
+                   "epochs": args.epochs, "loss": loss, "margin_log2": margin,
+                   "quantile": args.quantile, "collect_s": t1 - t0, "train_s": t2 - t1,
+                   "retried_rows": retried, "rows": ntiles,
+                   "bitwise_equal_to_plain": same}
+
+    if learned is not None:
+        ms_step, kern, work, sched = learned["ms_per_step"], learned["kernels_ms"], \
+            learned["walk_work"], "learned"
+        walk_kernel = (f"k_walk_rows<{kc}> capped pass + retry pass "
+                       "(+ k_mlp_predict, k_walk_exact)")
     else:
-        walk_kernel = (f"k_walk_dense<{kc}> + k_walk_sparse<{kc}> "
-                       "(+ k_walk_exact for deferred sources)")
-    traffic, traffic_src = pmc_traffic(cfg["name"]) if split == -1 and args.mode == 0 else (None, None)
-    achieved = bwalk / t_walk / 1e9
+        ms_step, kern, work, sched = ms_plain, kern_plain, work_plain, "plain"
+        if args.mode == 1 or kc == 0:
+            walk_kernel = "k_walk_exact"
+        elif split == -1:
+            walk_kernel = f"k_walk_rows<{kc}> (+ k_walk_exact for deferred sources)"
+        else:
+            walk_kernel = (f"k_walk_dense<{kc}> + k_walk_sparse<{kc}> "
+                           "(+ k_walk_exact for deferred sources)")
+    t_walk, achieved = roof(kern)
+    traffic, traffic_src = (pmc_traffic(cfg["name"], sched)
+                            if split == -1 and args.mode == 0 else (None, None))
     value = n_act / (ms_step * 1e-3)
     out = {
         "metric": METRIC,
@@ -317,9 +405,9 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded generator, see linkcell_amd/workloads.py)",
         "config": {"workload": cfg["name"], "n": n, "k": k, "cell_hint": hint,
-                   "mesh": [plan["nx"], plan["ny"], plan["nz"]], "schedule": "plain",
+                   "mesh": [plan["nx"], plan["ny"], plan["nz"]], "schedule": sched,
                    "mode": "auto" if args.mode == 0 else "exact-only"},
-        "walk_ms": kern["walk"] + kern["exact"],
+        "walk_ms": t_walk * 1e3,
         "walk_qps": n_act / t_walk,
         "kernels_ms": kern,
         "walk_work": work,
@@ -328,7 +416,11 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": bwalk,
                      "kernel": walk_kernel},
+        "plain": plain,
     }
+    if learned is not None:
+        out["learned"] = learned
+        out["learned_over_plain"] = ms_plain / ms_step
     if traffic_src:
         out["roofline"]["traffic_source"] = traffic_src
     if args.copy_calib:
@@ -337,6 +429,11 @@ def main():
         log("bench: cpu baseline")
         out["cpu_baseline"] = cpu_baseline(xyz, cell, k, hint, mask)
     print(json.dumps(out), flush=True)
+
+
+def wl_device_frame(f, dev):
+    from linkcell_amd import workloads as wl
+    return wl.clustered_c3_device(f, dev)
 
 
 if __name__ == "__main__":

@@ -52,6 +52,13 @@ int lc_gpu_set_mode(lc_gpu_workspace *ws, int mode);
 int lc_gpu_set_timing(lc_gpu_workspace *ws, int on);
 int lc_gpu_last_timing(lc_gpu_workspace *ws, double *ms7, int *deferred);
 
+/* The same timings split further, up to n values: fold, scan, scatter,
+ * cell sort, walk (all walk kernels), exact walk, total, then the parts of
+ * walk -- schedule set-up (learned caps), first pass, retry pass of rows
+ * whose cap was too small -- then the deferred-source and retried-row
+ * counts. */
+int lc_gpu_last_timing_ex(lc_gpu_workspace *ws, double *ms, int n);
+
 /* Many uneven systems (different n, cell, mesh and k) in one pipeline:
  * one launch per kernel for the whole batch. Errors name the frame. */
 int lc_gpu_knearest_batch(lc_gpu_workspace *ws, const lc_gpu_frame *frames,
@@ -83,9 +90,9 @@ int lc_gpu_set_dense_min(lc_gpu_workspace *ws, int dense_min);
 
 /* Walk schedule. Results are identical under every schedule.
  *   LC_GPU_SCHED_CELL     plain walk (the default)
- *   LC_GPU_SCHED_LEARNED  the workspace's trained predictor gives each home
- *                         cell a k-th-neighbour distance (plain walk until a
- *                         predictor is trained or loaded)
+ *   LC_GPU_SCHED_LEARNED  the workspace's trained predictor gives each source
+ *                         tile a k-th-neighbour distance (row walk only;
+ *                         plain walk until a predictor is trained or set)
  *   LC_GPU_SCHED_GIVEN    the per-cell distances set with
  *                         lc_gpu_sched_set_prediction
  * The distance caps the walk's pruning threshold from the first tile on; at
@@ -104,6 +111,72 @@ int lc_gpu_set_schedule(lc_gpu_workspace *ws, int mode);
  * pointer; it must stay valid while searches use it. */
 int lc_gpu_sched_set_prediction(lc_gpu_workspace *ws, const float *kth_dist,
                                 size_t n_cells);
+
+/* Per source tile instead of per cell, in the workspace's tile order (16
+ * consecutive sources of one home cell in the walk's cell-sorted order, as
+ * numbered by lc_gpu_sched_tile_labels); takes precedence over a per-cell
+ * prediction until lc_gpu_sched_set_prediction is called again. Tiles at or
+ * beyond n_tiles are not capped. A row whose cap was too small is walked
+ * again without it, so results never change. */
+int lc_gpu_sched_set_tile_prediction(lc_gpu_workspace *ws, const float *kth_dist,
+                                     size_t n_tiles);
+
+/* Number of source tiles of the last search (waits for it). */
+int lc_gpu_sched_tile_count(lc_gpu_workspace *ws, size_t *n_tiles);
+
+/* For every source tile of the last search, the largest final
+ * k-th-neighbour distance among its sources, read back from that search's
+ * out_d2 and rounded up to float (+inf for a source with fewer than k
+ * neighbours, or without out_d2): a device array of at least
+ * lc_gpu_sched_tile_count floats. The outputs of the last search must still
+ * be valid. Waits for completion. */
+int lc_gpu_sched_tile_labels(lc_gpu_workspace *ws, float *out, size_t n);
+
+/* Rows of the last timed search that were walked again without their
+ * predicted cap (needs timing on, like lc_gpu_last_timing). */
+int lc_gpu_last_retried(lc_gpu_workspace *ws, int *retried);
+
+/* Learned scheduler (LC_GPU_SCHED_LEARNED). Every workspace holds one small
+ * MLP (inputs: the home cell's 5x5x5 occupancies and the tile's shape and
+ * place; 160 -> 64 -> 64 -> 1, BF16 weights on MFMA, FP32 accumulation) that
+ * predicts each source tile's largest final k-th-neighbour distance; the
+ * row walk uses prediction * 2^margin as its pruning cap. Training runs on
+ * the device with the library's own forward, backward and Adam kernels.
+ *
+ * lc_gpu_sched_collect appends the tiles of the last search (which must
+ * have had out_d2) to the training set, labelled by that search's own
+ * k-th distances; lc_gpu_sched_train initialises the network from seed and
+ * runs epochs of minibatch Adam (batch a multiple of 64), then sets the
+ * margin to the given quantile of (label - prediction) in log2 units. */
+#define LC_GPU_SCHED_PARAMS 14529
+int lc_gpu_sched_collect(lc_gpu_workspace *ws, size_t *n_added);
+int lc_gpu_sched_samples(lc_gpu_workspace *ws, size_t *n);
+int lc_gpu_sched_clear(lc_gpu_workspace *ws);
+int lc_gpu_sched_train(lc_gpu_workspace *ws, int epochs, int batch, double lr,
+                       unsigned seed, double quantile, double *last_loss,
+                       double *margin);
+
+/* Network parameters as LC_GPU_SCHED_PARAMS host floats: W1 (160 x 64,
+ * input-major), b1 (64), W2 (64 x 64, input-major), b2 (64), w3 (64), b3;
+ * set_params also sets the margin and marks the predictor trained. */
+int lc_gpu_sched_get_params(lc_gpu_workspace *ws, float *params, size_t n,
+                            double *margin);
+int lc_gpu_sched_set_params(lc_gpu_workspace *ws, const float *params,
+                            size_t n, double margin);
+
+/* The network on host data (inputs rounded to BF16 on the device):
+ * forward: y for n samples of 160 inputs; gradient: of the loss
+ * sum_i w_i (y_i - t_i)^2 / (2 sum_i w_i) at the current parameters, in the
+ * parameter layout above, plus the loss; adam: `steps` full-batch Adam steps
+ * (beta 0.9 / 0.999, eps 1e-8, moments from zero), returning the loss
+ * before the last step. */
+int lc_gpu_sched_forward(lc_gpu_workspace *ws, const float *x, size_t n,
+                         float *y);
+int lc_gpu_sched_gradient(lc_gpu_workspace *ws, const float *x, const float *t,
+                          const float *w, size_t n, float *grad, double *loss);
+int lc_gpu_sched_adam(lc_gpu_workspace *ws, const float *x, const float *t,
+                      const float *w, size_t n, int steps, double lr,
+                      double *loss);
 
 /* Device work counters of the walk kernels (diagnostics). When on, each
  * search accumulates them; lc_gpu_last_stats waits for the last search

@@ -128,12 +128,16 @@ class Workspace:
         self._check(self._lib.lc_gpu_set_timing(self._ws, 1 if on else 0))
 
     def last_timing(self) -> dict:
-        ms = (C.c_double * 7)()
-        deferred = C.c_int(0)
-        self._check(self._lib.lc_gpu_last_timing(self._ws, ms, C.byref(deferred)))
-        keys = ("fold", "scan", "scatter", "sort", "walk", "exact", "total")
+        """Milliseconds per pipeline stage of the last timed search (walk
+        split into schedule set-up, first pass and retry pass), plus the
+        deferred-source and retried-row counts."""
+        ms = (C.c_double * 12)()
+        self._check(self._lib.lc_gpu_last_timing_ex(self._ws, ms, 12))
+        keys = ("fold", "scan", "scatter", "sort", "walk", "exact", "total",
+                "setup", "pass1", "retry")
         out = {k: float(v) for k, v in zip(keys, ms)}
-        out["deferred"] = int(deferred.value)
+        out["deferred"] = int(ms[10])
+        out["retried"] = int(ms[11])
         return out
 
     def set_dense_min(self, v: int):
@@ -151,6 +155,97 @@ class Workspace:
         self._check(self._lib.lc_gpu_sched_set_prediction(
             self._ws, C.c_void_p(kth_ptr or None), int(n_cells)))
 
+    def sched_set_tile_prediction(self, kth_ptr: int, n_tiles: int):
+        """Per-tile k-th-neighbour distances (device float32 array, the
+        workspace's tile order) for SCHED_GIVEN."""
+        self._check(self._lib.lc_gpu_sched_set_tile_prediction(
+            self._ws, C.c_void_p(kth_ptr or None), int(n_tiles)))
+
+    def tile_count(self) -> int:
+        """Source tiles of the last search."""
+        n = C.c_size_t(0)
+        self._check(self._lib.lc_gpu_sched_tile_count(self._ws, C.byref(n)))
+        return int(n.value)
+
+    def tile_labels(self, out_ptr: int, n: int):
+        """Largest final k-th-neighbour distance per source tile of the last
+        search into a device float32 array."""
+        self._check(self._lib.lc_gpu_sched_tile_labels(self._ws, C.c_void_p(out_ptr), int(n)))
+
+    def last_retried(self) -> int:
+        """Rows of the last timed search walked again without their cap."""
+        r = C.c_int(0)
+        self._check(self._lib.lc_gpu_last_retried(self._ws, C.byref(r)))
+        return int(r.value)
+
+    # learned scheduler ------------------------------------------------
+    SCHED_PARAMS = 14529
+
+    def sched_collect(self) -> int:
+        """Append the source tiles of the last search (run with out_d2) to
+        the predictor's training set; returns the samples added."""
+        n = C.c_size_t(0)
+        self._check(self._lib.lc_gpu_sched_collect(self._ws, C.byref(n)))
+        return int(n.value)
+
+    def sched_samples(self) -> int:
+        n = C.c_size_t(0)
+        self._check(self._lib.lc_gpu_sched_samples(self._ws, C.byref(n)))
+        return int(n.value)
+
+    def sched_clear(self):
+        self._check(self._lib.lc_gpu_sched_clear(self._ws))
+
+    def sched_train(self, epochs=4, batch=4096, lr=2e-3, seed=1, quantile=0.98):
+        """Train the predictor on the collected set; returns (last epoch's
+        mean loss, margin)."""
+        loss = C.c_double(0.0)
+        margin = C.c_double(0.0)
+        self._check(self._lib.lc_gpu_sched_train(
+            self._ws, int(epochs), int(batch), float(lr), int(seed) & 0xFFFFFFFF,
+            float(quantile), C.byref(loss), C.byref(margin)))
+        return float(loss.value), float(margin.value)
+
+    def sched_get_params(self):
+        p = np.zeros(self.SCHED_PARAMS, dtype=np.float32)
+        margin = C.c_double(0.0)
+        self._check(self._lib.lc_gpu_sched_get_params(
+            self._ws, p.ctypes.data, p.size, C.byref(margin)))
+        return p, float(margin.value)
+
+    def sched_set_params(self, params, margin=0.0):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self._lib.lc_gpu_sched_set_params(
+            self._ws, p.ctypes.data, p.size, float(margin)))
+
+    def sched_forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros(x.shape[0], dtype=np.float32)
+        self._check(self._lib.lc_gpu_sched_forward(self._ws, x.ctypes.data, x.shape[0],
+                                                   y.ctypes.data))
+        return y
+
+    def sched_gradient(self, x, t, w):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        g = np.zeros(self.SCHED_PARAMS, dtype=np.float32)
+        loss = C.c_double(0.0)
+        self._check(self._lib.lc_gpu_sched_gradient(
+            self._ws, x.ctypes.data, t.ctypes.data, w.ctypes.data, x.shape[0],
+            g.ctypes.data, C.byref(loss)))
+        return g, float(loss.value)
+
+    def sched_adam(self, x, t, w, steps, lr):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        loss = C.c_double(0.0)
+        self._check(self._lib.lc_gpu_sched_adam(
+            self._ws, x.ctypes.data, t.ctypes.data, w.ctypes.data, x.shape[0],
+            int(steps), float(lr), C.byref(loss)))
+        return float(loss.value)
+
     def set_stats(self, on: bool):
         self._check(self._lib.lc_gpu_set_stats(self._ws, 1 if on else 0))
 
@@ -158,7 +253,8 @@ class Workspace:
                   "dense_merges", "dense_appends", "sparse_sources",
                   "sparse_tile_tests", "sparse_tiles", "sparse_evals",
                   "sparse_merges", "row_sources", "row_tile_tests", "row_tiles",
-                  "row_merges", "row_rounds", "row_cells", "row_refines")
+                  "row_merges", "row_rounds", "row_cells", "row_refines",
+                  "row_useful")
 
     def last_stats(self) -> dict:
         n = len(self.STAT_NAMES)

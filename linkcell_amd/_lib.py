@@ -26,6 +26,12 @@ EXPORTS = (
     "lc_gpu_last_stats", "lc_gpu_knearest_batch", "lc_gpu_graph_capture",
     "lc_gpu_graph_launch", "lc_gpu_graph_reset", "lc_gpu_autotune",
     "lc_gpu_set_schedule", "lc_gpu_sched_set_prediction",
+    "lc_gpu_sched_set_tile_prediction", "lc_gpu_sched_tile_count",
+    "lc_gpu_sched_tile_labels", "lc_gpu_last_retried",
+    "lc_gpu_sched_collect", "lc_gpu_sched_samples", "lc_gpu_sched_clear",
+    "lc_gpu_sched_train", "lc_gpu_sched_get_params", "lc_gpu_sched_set_params",
+    "lc_gpu_sched_forward", "lc_gpu_sched_gradient", "lc_gpu_sched_adam",
+    "lc_gpu_last_timing_ex",
 )
 
 
@@ -94,6 +100,21 @@ _PROTOS = {
     "lc_gpu_autotune": (C.c_int, [_vp, C.POINTER(lc_gpu_frame), _sz, _ip, _dp]),
     "lc_gpu_set_schedule": (C.c_int, [_vp, C.c_int]),
     "lc_gpu_sched_set_prediction": (C.c_int, [_vp, _vp, _sz]),
+    "lc_gpu_sched_set_tile_prediction": (C.c_int, [_vp, _vp, _sz]),
+    "lc_gpu_sched_tile_count": (C.c_int, [_vp, C.POINTER(_sz)]),
+    "lc_gpu_sched_tile_labels": (C.c_int, [_vp, _vp, _sz]),
+    "lc_gpu_last_retried": (C.c_int, [_vp, _ip]),
+    "lc_gpu_last_timing_ex": (C.c_int, [_vp, _dp, C.c_int]),
+    "lc_gpu_sched_collect": (C.c_int, [_vp, C.POINTER(_sz)]),
+    "lc_gpu_sched_samples": (C.c_int, [_vp, C.POINTER(_sz)]),
+    "lc_gpu_sched_clear": (C.c_int, [_vp]),
+    "lc_gpu_sched_train": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_uint,
+                                     C.c_double, _dp, _dp]),
+    "lc_gpu_sched_get_params": (C.c_int, [_vp, _vp, _sz, _dp]),
+    "lc_gpu_sched_set_params": (C.c_int, [_vp, _vp, _sz, C.c_double]),
+    "lc_gpu_sched_forward": (C.c_int, [_vp, _vp, _sz, _vp]),
+    "lc_gpu_sched_gradient": (C.c_int, [_vp, _vp, _vp, _vp, _sz, _vp, _dp]),
+    "lc_gpu_sched_adam": (C.c_int, [_vp, _vp, _vp, _vp, _sz, C.c_int, C.c_double, _dp]),
 }
 
 _lib = None

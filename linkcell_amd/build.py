@@ -17,7 +17,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "liblinkcell_amd.so"
 
-SOURCES = ["lc_engine.hip", "lc_api.hip"]
+SOURCES = ["lc_engine.hip", "lc_sched.hip", "lc_api.hip"]
 
 # -ffp-contract=off: no FMA contraction anywhere, so every double
 # operation rounds exactly as the reference CPU walk's.

@@ -7,6 +7,7 @@
 //   lc_gpu_knearest_ex and the timing / mode extensions
 #include "../../include/linkcell_amd.h"
 #include "lc_engine.hpp"
+#include "lc_sched.hpp"
 #include "../../include/linkcell_gpu.h"
 #include "../../include/linkcell_gpu.hpp"
 #include "lc_cell.hpp"
@@ -362,6 +363,21 @@ int lc_gpu_last_timing(lc_gpu_workspace* ws, double* ms7, int* deferred) {
   });
 }
 
+int lc_gpu_last_timing_ex(lc_gpu_workspace* ws, double* ms, int n) {
+  if (ws == nullptr || ms == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    const lcamd::Timings& t = ws->ws.impl()->engine.last_timings();
+    const double v[12] = {t.fold,  t.scan,  t.scatter, t.sort,
+                          t.walk,  t.exact, t.total,   t.setup,
+                          t.pass1, t.retry, static_cast<double>(t.deferred),
+                          static_cast<double>(t.retried)};
+    for (int i = 0; i < n; ++i) ms[i] = i < 12 ? v[i] : 0.0;
+  });
+}
+
 }  // extern "C"
 
 namespace {
@@ -518,6 +534,191 @@ int lc_gpu_sched_set_prediction(lc_gpu_workspace* ws, const float* kth_dist,
                                              static_cast<long long>(n_cells));
   set_gpu_err(nullptr);
   return 0;
+}
+
+int lc_gpu_sched_set_tile_prediction(lc_gpu_workspace* ws,
+                                     const float* kth_dist, size_t n_tiles) {
+  if (ws == nullptr || (kth_dist == nullptr && n_tiles != 0)) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  ws->ws.impl()->engine.set_given_tile_prediction(
+      kth_dist, static_cast<long long>(n_tiles));
+  set_gpu_err(nullptr);
+  return 0;
+}
+
+int lc_gpu_sched_tile_count(lc_gpu_workspace* ws, size_t* n_tiles) {
+  if (ws == nullptr || n_tiles == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    *n_tiles = static_cast<size_t>(ws->ws.impl()->engine.tile_count());
+  });
+}
+
+int lc_gpu_sched_tile_labels(lc_gpu_workspace* ws, float* out, size_t n) {
+  if (ws == nullptr || out == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    try {
+      ws->ws.impl()->engine.tile_labels(out, static_cast<long long>(n));
+    } catch (const std::runtime_error& e) {
+      throw linkcell::Error(e.what());
+    }
+  });
+}
+
+int lc_gpu_last_retried(lc_gpu_workspace* ws, int* retried) {
+  if (ws == nullptr || retried == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    *retried = ws->ws.impl()->engine.last_timings().retried;
+  });
+}
+
+int lc_gpu_sched_collect(lc_gpu_workspace* ws, size_t* n_added) {
+  if (ws == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    const long long a = ws->ws.impl()->engine.sched_collect();
+    if (n_added) *n_added = static_cast<size_t>(a);
+  });
+}
+
+int lc_gpu_sched_samples(lc_gpu_workspace* ws, size_t* n) {
+  if (ws == nullptr || n == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    *n = static_cast<size_t>(ws->ws.impl()->engine.predictor().samples());
+  });
+}
+
+int lc_gpu_sched_clear(lc_gpu_workspace* ws) {
+  if (ws == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] { ws->ws.impl()->engine.predictor().clear_samples(); });
+}
+
+int lc_gpu_sched_train(lc_gpu_workspace* ws, int epochs, int batch, double lr,
+                       unsigned seed, double quantile, double* last_loss,
+                       double* margin) {
+  if (ws == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    lcamd::Engine& e = ws->ws.impl()->engine;
+    try {
+      const double l = e.predictor().train(epochs, batch, lr, seed, quantile, e.stream());
+      if (last_loss) *last_loss = l;
+      if (margin) *margin = e.predictor().margin();
+    } catch (const std::runtime_error& err) {
+      throw linkcell::Error(err.what());
+    }
+  });
+}
+
+int lc_gpu_sched_get_params(lc_gpu_workspace* ws, float* params, size_t n,
+                            double* margin) {
+  if (ws == nullptr || params == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  if (n != static_cast<size_t>(lcamd::kParams)) {
+    set_gpu_err("parameter count must be LC_GPU_SCHED_PARAMS");
+    return 1;
+  }
+  return guarded([&] {
+    lcamd::Predictor& p = ws->ws.impl()->engine.predictor();
+    try {
+      p.get_params(params);
+    } catch (const std::runtime_error& err) {
+      throw linkcell::Error(err.what());
+    }
+    if (margin) *margin = p.margin();
+  });
+}
+
+int lc_gpu_sched_set_params(lc_gpu_workspace* ws, const float* params, size_t n,
+                            double margin) {
+  if (ws == nullptr || params == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  if (n != static_cast<size_t>(lcamd::kParams)) {
+    set_gpu_err("parameter count must be LC_GPU_SCHED_PARAMS");
+    return 1;
+  }
+  return guarded([&] {
+    lcamd::Engine& e = ws->ws.impl()->engine;
+    e.predictor().set_params(params, margin, e.stream());
+  });
+}
+
+int lc_gpu_sched_forward(lc_gpu_workspace* ws, const float* x, size_t n,
+                         float* y) {
+  if (ws == nullptr || x == nullptr || y == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    lcamd::Engine& e = ws->ws.impl()->engine;
+    try {
+      e.predictor().forward_host(x, static_cast<long long>(n), y, e.stream());
+    } catch (const std::runtime_error& err) {
+      throw linkcell::Error(err.what());
+    }
+  });
+}
+
+int lc_gpu_sched_gradient(lc_gpu_workspace* ws, const float* x, const float* t,
+                          const float* w, size_t n, float* grad, double* loss) {
+  if (ws == nullptr || x == nullptr || t == nullptr || w == nullptr ||
+      grad == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    lcamd::Engine& e = ws->ws.impl()->engine;
+    try {
+      const double l = e.predictor().grad_host(x, t, w, static_cast<long long>(n),
+                                               grad, e.stream());
+      if (loss) *loss = l;
+    } catch (const std::runtime_error& err) {
+      throw linkcell::Error(err.what());
+    }
+  });
+}
+
+int lc_gpu_sched_adam(lc_gpu_workspace* ws, const float* x, const float* t,
+                      const float* w, size_t n, int steps, double lr,
+                      double* loss) {
+  if (ws == nullptr || x == nullptr || t == nullptr || w == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    lcamd::Engine& e = ws->ws.impl()->engine;
+    try {
+      const double l = e.predictor().adam_host(x, t, w, static_cast<long long>(n),
+                                               steps, lr, e.stream());
+      if (loss) *loss = l;
+    } catch (const std::runtime_error& err) {
+      throw linkcell::Error(err.what());
+    }
+  });
 }
 
 int lc_gpu_last_stats(lc_gpu_workspace* ws, unsigned long long* out, int n) {

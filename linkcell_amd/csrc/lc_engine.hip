@@ -30,6 +30,7 @@
 // Reference: src/knearest.rs:351-486 (the CPU walk, the parity target) and
 // src/gpu/knearest.cpp:192-441 (the CUDA kernels this file replaces).
 #include "lc_engine.hpp"
+#include "lc_sched.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -679,9 +680,18 @@ struct WalkArgs {
   const int* sched;
   const int* sched_count;  // kSchedBuckets run lengths
   const int* tile_cell;    // home cell of every 16-slot source tile
-  // optional predicted k-th-neighbour distance per home cell (null = plain
-  // walk): the row walk's starting threshold
+  // optional predicted k-th-neighbour distance per home cell and per
+  // source tile (null = plain walk): the row walk's threshold cap
   const float* pred;
+  const float* pred_tile;
+  long long pred_tile_n;  // tiles beyond it are not capped
+  // capped row walk: a row some of whose sources the cap may have kept
+  // from the reference's stop sets its tile's flag here and emits nothing
+  // (null: such sources go to the exact-order walk instead)
+  unsigned char* retry;
+  int* retry_count;
+  // retry pass: only the rows whose tile flag is set walk (null = all)
+  const unsigned char* only;
 };
 
 enum {
@@ -703,6 +713,7 @@ enum {
   kStatRowRounds,   // wave-level tile rounds of the row walk
   kStatRowCells,    // wave-level candidate-cell visits of the row walk
   kStatRowRefines,  // row-level exact tile refinements
+  kStatRowUseful,   // row-level tiles swept (the rest of the wave's rounds idle)
   kStatCount = 24
 };
 
@@ -1315,8 +1326,12 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   const int nt_frame = A.tile_off[A.fcb[f + 1]] - tb;
   const int tf = (wg - row_wave0(A.tile_off, A.fcb, f)) * 4 + rw;
   if (__ballot(tf < nt_frame) == 0) return;
-  const bool has_tile = tf < nt_frame;
+  bool has_tile = tf < nt_frame;
   const int T = tb + (has_tile ? tf : 0);
+  if (A.only != nullptr) {
+    has_tile = has_tile && A.only[T] != 0;
+    if (__ballot(has_tile) == 0) return;
+  }
   const int g = A.tile_cell[T];
   const int o0 = A.off[g];
   const int cnt = A.off[g + 1] - o0;
@@ -1389,8 +1404,13 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     const double q = static_cast<double>(A.pred[g]);
     thr_cap = q * q;
   }
+  if (A.pred_tile != nullptr && T < A.pred_tile_n) {
+    const double q = static_cast<double>(A.pred_tile[T]);
+    thr_cap = fmin(thr_cap, q * q);
+  }
   double thr = thr_cap;
   bool live = valid;
+  bool cap_fail = false;
   double rthr = red16_max(live ? thr : -inf);  // -inf: no live source
   int nbuf = 0;
   int done_reach = 0;
@@ -1399,7 +1419,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   int* lslot = &s_slot[wv][rw][0];
   int* lmeta = &s_meta[wv][rw][0];
   unsigned st_tests = 0, st_tiles = 0, st_merges = 0, st_rounds = 0,
-           st_cells = 0, st_exact = 0;
+           st_cells = 0, st_exact = 0, st_useful = 0;
 
   auto flush = [&]() {
     if (nbuf > 0) {
@@ -1506,6 +1526,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       cy = nyq + shy;
       cz = nzq + shz;
       cj = njq;
+      if (kStats) st_useful += i < lcount ? 1 : 0;
       i += 1;
       fetch(i);
       if (kStats) st_tiles += 1;
@@ -1676,8 +1697,9 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       }
     } else if (live && thr_cap < bound * bound) {
       // not full: the plain walk might have filled its heap with candidates
-      // beyond the cap and stopped here; only the exact walk can tell
+      // beyond the cap and stopped here; only an uncapped walk can tell
       live = false;
+      cap_fail = true;
     }
     // beyond r_safe the stencil wraps onto itself: exact walk
     if (live && r + 1 > d.r_safe) live = false;
@@ -1691,10 +1713,23 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     if (li == 0) {
       stat_add(A.stats, kStatRowTiles, st_tiles);
       stat_add(A.stats, kStatRowRefines, st_exact);
+      stat_add(A.stats, kStatRowUseful, st_useful);
     }
     if (lane == 0) {
       stat_add(A.stats, kStatRowRounds, st_rounds);
       stat_add(A.stats, kStatRowCells, st_cells);
+    }
+  }
+  if (A.retry != nullptr) {
+    // a row with a cap failure is walked again, whole, without the cap by
+    // the retry pass; none of its sources emits or defers now
+    const unsigned long long fb = __ballot(cap_fail);
+    if (((fb >> rbase) & 0xffffull) != 0) {
+      if (li == 0 && has_tile) {
+        A.retry[T] = 1;
+        atomicAdd(A.retry_count, 1);
+      }
+      return;
     }
   }
   if (!valid) return;
@@ -1772,6 +1807,30 @@ __global__ __launch_bounds__(256) void k_sched(
       }
     }
   }
+}
+
+// Label of every source tile of a finished search: the largest final
+// k-th-neighbour distance among the tile's sources, read back from the
+// frame's out_d2 and rounded up to float (+inf where a source has fewer than
+// k neighbours or out_d2 is absent). out[] starts at zero; non-negative
+// floats order like their bit patterns.
+__global__ __launch_bounds__(256) void k_tile_labels(
+    const FrameDesc* __restrict__ fr, const long long* __restrict__ fcb, int F,
+    const Cand* __restrict__ cand, const int* __restrict__ scell,
+    const int* __restrict__ off, const int* __restrict__ tile_off,
+    long long ncells, int* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= off[ncells]) return;
+  const int g = scell[s];
+  const FrameDesc& d = fr[find_frame(fcb, F, g)];
+  const int T = tile_off[g] + ((s - off[g]) >> kTileShift);
+  float v = __builtin_inff();
+  if (d.out_d2 != nullptr) {
+    const int i = static_cast<int>(cand[s].id);
+    const double kth = d.out_d2[static_cast<size_t>(i) * d.k + d.k - 1];
+    if (kth == kth) v = __double2float_ru(sqrt(kth));
+  }
+  atomicMax(&out[T], __float_as_int(v));
 }
 
 // ---------------------------------------------------------- exact walk
@@ -2014,7 +2073,8 @@ void Engine::free_all() {
                 &cellslot_, &tmp_key_,  &cand_,
                 &scell_,    &byidx_,  &tbox_,   &cbox_,
                 &defer_,    &heap_d2_,  &heap_idx_, &flags_, &stats_,
-                &sched_,    &sched_cnt_, &tile_cell_};
+                &sched_,    &sched_cnt_, &tile_cell_, &retry_,
+                &pred_tile_, &labels_};
   for (Buf* b : all) {
     if (b->p != nullptr) (void)hipFree(b->p);
     b->p = nullptr;
@@ -2037,11 +2097,19 @@ const Timings& Engine::last_timings() {
     timings_.walk = el(4, 5);
     timings_.exact = el(5, 6);
     timings_.total = el(0, 7);
+    timings_.setup = el(4, 8);
+    timings_.pass1 = el(8, 9);
+    timings_.retry = el(9, 5);
     int dc = 0;
     check(hipMemcpy(&dc, as<int>(counters_.p) + 1, sizeof(int),
                     hipMemcpyDeviceToHost),
           "DtoH deferred");
     timings_.deferred = dc;
+    int rt = 0;
+    check(hipMemcpy(&rt, as<int>(counters_.p) + 3, sizeof(int),
+                    hipMemcpyDeviceToHost),
+          "DtoH retried");
+    timings_.retried = rt;
     timings_pending_ = false;
   }
   return timings_;
@@ -2094,6 +2162,9 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   if (ntot >= INT_MAX / 2 || ctot >= INT_MAX / 2) {
     throw std::runtime_error("batch exceeds 2^30 points or cells");
   }
+  last_F_ = F;
+  last_ntot_ = ntot;
+  last_ctot_ = ctot;
   const long long tiles = ctot / kScanTile + 1;
   // 16-point tiles: at most one partial tile per non-empty cell
   const long long tcap = ntot / kTile + std::min(ntot, ctot) + 1;
@@ -2269,15 +2340,46 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   wa.sched = nullptr;
   wa.sched_count = nullptr;
   wa.pred = nullptr;
+  wa.pred_tile = nullptr;
+  wa.pred_tile_n = 0;
+  wa.retry = nullptr;
+  wa.retry_count = nullptr;
+  wa.only = nullptr;
   const float* pred = nullptr;
+  if (sched_mode_ == 1 && pred_ != nullptr && pred_->trained() &&
+      wa.dense_min == kRowWalk) {
+    // learned caps of every source tile, from the counts and tile boxes
+    // just built
+    grow(pred_tile_, sizeof(float) * static_cast<size_t>(tcap));
+    pred_->predict_caps(tile_src(ctot, F), tcap, as<float>(pred_tile_.p), st);
+    wa.pred_tile = as<float>(pred_tile_.p);
+    wa.pred_tile_n = tcap;
+  }
   if (sched_mode_ == 2) {
-    if (given_pred_ == nullptr || given_pred_n_ != ctot) {
-      throw std::runtime_error(
-          "schedule prediction length must equal the batch's cell count");
+    if (given_tile_ != nullptr) {
+      wa.pred_tile = given_tile_;
+      wa.pred_tile_n = given_tile_n_;
+    } else {
+      if (given_pred_ == nullptr || given_pred_n_ != ctot) {
+        throw std::runtime_error(
+            "schedule prediction length must equal the batch's cell count");
+      }
+      pred = given_pred_;
     }
-    pred = given_pred_;
   }
   wa.pred = pred;
+  // capped row walk: rows whose cap may have hidden the reference's stop
+  // are walked again without it (second launch per heap bucket)
+  const bool capped =
+      (wa.pred != nullptr || wa.pred_tile != nullptr) && wa.dense_min == kRowWalk;
+  if (capped) {
+    const long long words = (tcap + 3) / 4;
+    grow(retry_, sizeof(unsigned) * static_cast<size_t>(words));
+    k_zero_u32<<<blocks_for(words, 256), 256, 0, st>>>(as<unsigned>(retry_.p),
+                                                      words);
+    wa.retry = as<unsigned char>(retry_.p);
+    wa.retry_count = counters + 3;
+  }
   const float* depth = pred;
   if (depth != nullptr && wa.dense_min != kRowWalk) {
     grow(sched_, sizeof(int) * static_cast<size_t>(ntot));
@@ -2297,6 +2399,14 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   const unsigned sparse_blocks = blocks_for(ntot, 256);
   // row walk: one wave per 4 source tiles, frames on wave boundaries
   const unsigned row_blocks = blocks_for(tcap / 4 + F + 1, 4);
+  WalkArgs wr = wa;  // retry pass: uncapped, flagged rows only
+  wr.pred = nullptr;
+  wr.pred_tile = nullptr;
+  wr.pred_tile_n = 0;
+  wr.retry = nullptr;
+  wr.retry_count = nullptr;
+  wr.only = wa.retry;
+  if (timing) check(hipEventRecord(ev_[8], st), "event record");
 #define LC_WALK(KC)                                                           \
   if (any_fast[KC] && wa.dense_min == kRowWalk) {                             \
     if (wa.stats != nullptr) {                                                \
@@ -2317,6 +2427,22 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   LC_WALK(16)
   LC_WALK(32)
 #undef LC_WALK
+  if (timing) check(hipEventRecord(ev_[9], st), "event record");
+  // retry pass of the capped row walk: the flagged rows, uncapped
+#define LC_RETRY(KC)                                                          \
+  if (capped && any_fast[KC]) {                                               \
+    if (wa.stats != nullptr) {                                                \
+      k_walk_rows<KC, true><<<row_blocks, 256, 0, st>>>(wr);                  \
+    } else {                                                                  \
+      k_walk_rows<KC, false><<<row_blocks, 256, 0, st>>>(wr);                 \
+    }                                                                         \
+    check(hipGetLastError(), "retry walk launch");                            \
+  }
+  LC_RETRY(4)
+  LC_RETRY(8)
+  LC_RETRY(16)
+  LC_RETRY(32)
+#undef LC_RETRY
   // frames outside every register bucket go straight to the exact walk
   bool any_exact_frame = false;
   for (int f = 0; f < F; ++f) {
@@ -2418,6 +2544,59 @@ void Engine::reset_graph() {
     (void)hipGraphExecDestroy(graph_exec_);
     graph_exec_ = nullptr;
   }
+}
+
+TileSrc Engine::tile_src(long long ctot, int F) {
+  TileSrc t;
+  t.fr = as<FrameDesc>(frames_.p);
+  t.fcb = as<long long>(fcb_.p);
+  t.F = F;
+  t.count = as<int>(count_.p);
+  t.off = as<int>(off_.p);
+  t.tile_off = as<int>(tile_off_.p);
+  t.tile_cell = as<int>(tile_cell_.p);
+  t.tbox = as<double>(tbox_.p);
+  t.ncells = ctot;
+  return t;
+}
+
+Predictor& Engine::predictor() {
+  if (pred_ == nullptr) pred_ = std::make_unique<Predictor>();
+  return *pred_;
+}
+
+long long Engine::sched_collect() {
+  const long long nt = tile_count();
+  if (nt == 0) return 0;
+  grow(labels_, sizeof(float) * static_cast<size_t>(nt));
+  tile_labels(as<float>(labels_.p), nt);
+  return predictor().collect(tile_src(last_ctot_, last_F_), nt,
+                             as<float>(labels_.p), stream());
+}
+
+long long Engine::tile_count() {
+  if (last_F_ == 0) return 0;
+  sync();
+  int t = 0;
+  check(hipMemcpy(&t, as<int>(tile_off_.p) + last_ctot_, sizeof(int),
+                  hipMemcpyDeviceToHost),
+        "DtoH tile count");
+  return t;
+}
+
+void Engine::tile_labels(float* out, long long n) {
+  const long long nt = tile_count();
+  if (n < nt) {
+    throw std::runtime_error("label buffer shorter than the search's tile count");
+  }
+  hipStream_t st = stream();
+  k_zero_u32<<<blocks_for(nt, 256), 256, 0, st>>>(reinterpret_cast<unsigned*>(out), nt);
+  k_tile_labels<<<blocks_for(last_ntot_, 256), 256, 0, st>>>(
+      as<FrameDesc>(frames_.p), as<long long>(fcb_.p), last_F_, as<Cand>(cand_.p),
+      as<int>(scell_.p), as<int>(off_.p), as<int>(tile_off_.p), last_ctot_,
+      reinterpret_cast<int*>(out));
+  check(hipGetLastError(), "tile label launch");
+  sync();
 }
 
 int Engine::autotune(const std::vector<FrameSpec>& frames, float* best_ms) {

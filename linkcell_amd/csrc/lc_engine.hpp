@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
@@ -41,6 +42,22 @@ struct FrameDesc {
   int kc;  // register-heap bucket for the fast walk, 0 = exact walk only
 };
 
+// Where the source tiles of a search live (device pointers of the engine);
+// read by the learned scheduler's feature map.
+struct TileSrc {
+  const FrameDesc* fr;
+  const long long* fcb;
+  int F;
+  const int* count;      // occupancy per cell of the batch
+  const int* off;        // cell offsets (ncells + 1)
+  const int* tile_off;   // first tile per cell (ncells + 1)
+  const int* tile_cell;  // home cell per tile
+  const double* tbox;    // per tile: lx, hx, ly, hy, lz, hz, pad, pad
+  long long ncells;
+};
+
+class Predictor;
+
 // Host description of one frame of a call.
 struct FrameSpec {
   const double* xyz = nullptr;
@@ -59,7 +76,11 @@ struct FrameSpec {
 struct Timings {
   float fold = 0, scan = 0, scatter = 0, sort = 0, walk = 0, exact = 0,
         total = 0;
+  // parts of walk: schedule set-up (learned caps, depth order), the first
+  // (capped) pass of every walk kernel, the retry pass of the capped rows
+  float setup = 0, pass1 = 0, retry = 0;
   int deferred = 0;  // sources finished by the exact-order walk
+  int retried = 0;   // rows walked again without their predicted cap
 };
 
 enum class WalkMode : int {
@@ -112,7 +133,27 @@ class Engine {
   void set_given_prediction(const float* kth, long long n) {
     given_pred_ = kth;
     given_pred_n_ = n;
+    given_tile_ = nullptr;
+    given_tile_n_ = 0;
   }
+  // Per source tile instead of per cell (the engine's tile order, see
+  // tile_labels); takes precedence over a per-cell prediction.
+  void set_given_tile_prediction(const float* kth, long long n) {
+    given_tile_ = kth;
+    given_tile_n_ = n;
+  }
+  // Source tiles (16 consecutive slots of one home cell in the walk's
+  // cell-sorted order) of the last search; waits for it.
+  long long tile_count();
+  // For every tile of the last search, the largest final k-th-neighbour
+  // distance among its sources (from the frames' out_d2), rounded up to
+  // float, into the device array out (at least tile_count() floats).
+  void tile_labels(float* out, long long n);
+  // The workspace's learned scheduler (created on first use).
+  Predictor& predictor();
+  // Append the source tiles of the last search (run with out_d2) to the
+  // predictor's training set; returns the samples added.
+  long long sched_collect();
   // Device work counters of the walks (diagnostics).
   void set_stats(bool on) { stats_on_ = on; }
   void read_stats(unsigned long long* out, int n);
@@ -130,10 +171,17 @@ class Engine {
   Buf frames_, fpt_, fcb_, count_, status_, counters_, off_, tile_off_,
       item_off_, items_, cellslot_, tmp_key_, cand_, scell_,
       byidx_, tbox_, cbox_, defer_, heap_d2_, heap_idx_, flags_, stats_,
-      sched_, sched_cnt_, tile_cell_;
+      sched_, sched_cnt_, tile_cell_, retry_;
   int sched_mode_ = 0;
   const float* given_pred_ = nullptr;
   long long given_pred_n_ = 0;
+  const float* given_tile_ = nullptr;
+  long long given_tile_n_ = 0;
+  int last_F_ = 0;
+  long long last_ntot_ = 0, last_ctot_ = 0;
+  std::unique_ptr<Predictor> pred_;
+  Buf pred_tile_, labels_;
+  TileSrc tile_src(long long ctot, int F);
   void* staging_ = nullptr;
   size_t staging_bytes_ = 0;
   hipEvent_t staging_ev_ = nullptr;
@@ -149,7 +197,7 @@ class Engine {
   std::vector<std::pair<std::string, int>> tuned_;
   bool stats_on_ = false;
   long long stats_n_ = 0;
-  hipEvent_t ev_[8] = {};
+  hipEvent_t ev_[10] = {};
   Timings timings_{};
 };
 

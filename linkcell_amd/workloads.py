@@ -12,7 +12,9 @@ device. Configurations (sizes fixed, seeds fixed):
                     vapour elsewhere, 2000 Gaussian clusters (sigma 1.5),
                     membership in proportion 172,800 : 6,912 : 400,000
                     (the stated densities and cluster sizes, scaled to N),
-                    k=16.
+                    k=16. ``clustered_c3_device`` draws further frames of the
+                    same mixture on the device (training frames);
+                    ``clustered_c3_small`` is the mixture at 1/16 scale.
 * ``batched_c4``    256 frames, N log-uniform in [1e3, 2e5], even frames
                     orthorhombic (edges 15-80), odd frames triclinic (tilts
                     up to 0.4 of the edge), uniform or clustered, k in
@@ -92,25 +94,68 @@ def triclinic_c2(n=262144):
 C3_BOX = (120.0, 120.0, 360.0)
 
 
-def clustered_c3(frame=0, n=4194304, n_clusters=2000):
+def clustered_c3(frame=0, n=4194304, n_clusters=2000, box=C3_BOX):
+    """The slab / vapour / cluster mixture. Smaller n with a box of
+    proportionally smaller volume and n_clusters keeps the same local
+    densities (the training and test frames of the learned scheduler)."""
     rng = np.random.default_rng(1000 + frame)
     w = np.array([172800.0, 6912.0, 400000.0])
     counts = rng.multinomial(n, w / w.sum())
     n_slab, n_vap, n_clu = (int(c) for c in counts)
-    lx, ly, lz = C3_BOX
+    lx, ly, lz = box
+    third = lz / 3.0
     slab = np.column_stack([rng.random(n_slab) * lx, rng.random(n_slab) * ly,
-                            120.0 + rng.random(n_slab) * 120.0])
-    vz = rng.random(n_vap) * 240.0
-    vz = np.where(vz < 120.0, vz, vz + 120.0)
+                            third + rng.random(n_slab) * third])
+    vz = rng.random(n_vap) * 2.0 * third
+    vz = np.where(vz < third, vz, vz + third)
     vap = np.column_stack([rng.random(n_vap) * lx, rng.random(n_vap) * ly, vz])
-    centres = rng.random((n_clusters, 3)) * np.array(C3_BOX)
+    centres = rng.random((n_clusters, 3)) * np.array(box)
     member = rng.integers(0, n_clusters, size=n_clu)
     clu = centres[member] + rng.normal(0.0, 1.5, size=(n_clu, 3))
     xyz = np.concatenate([slab, vap, clu])
     perm = rng.permutation(n)
     xyz = np.ascontiguousarray(xyz[perm])
-    return dict(name=f"clustered-slab-120x120x360-N{n}-k16", xyz=xyz,
-                cell=np.array(C3_BOX), k=16, hint=3.0, mask=None)
+    bx = "x".join(f"{v:g}" for v in box)
+    return dict(name=f"clustered-slab-{bx}-N{n}-k16", xyz=xyz,
+                cell=np.array(box, dtype=np.float64), k=16, hint=3.0, mask=None)
+
+
+def clustered_c3_device(frame, device, n=4194304, n_clusters=2000, box=C3_BOX):
+    """The clustered mixture drawn on the device with torch's generator
+    (seed 2000 + frame): the same distribution as clustered_c3, not the
+    same points. Used for the learned scheduler's training frames, where
+    32 frames of 4M points would take a minute with numpy."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(2000 + frame)
+    w = np.array([172800.0, 6912.0, 400000.0])
+    c = np.cumsum(w / w.sum())
+    v = torch.rand(n, dtype=torch.float64, device=device, generator=g)
+    comp = (v >= c[0]).to(torch.int64) + (v >= c[1]).to(torch.int64)
+    lx, ly, lz = box
+    third = lz / 3.0
+    u = torch.rand((n, 3), dtype=torch.float64, device=device, generator=g)
+    L = torch.tensor([lx, ly, lz], dtype=torch.float64, device=device)
+    xyz = u * L
+    slab = comp == 0
+    xyz[:, 2] = torch.where(slab, third + u[:, 2] * third, xyz[:, 2])
+    vz = u[:, 2] * 2.0 * third
+    vz = torch.where(vz < third, vz, vz + third)
+    xyz[:, 2] = torch.where(comp == 1, vz, xyz[:, 2])
+    centres = torch.rand((n_clusters, 3), dtype=torch.float64, device=device, generator=g) * L
+    member = torch.randint(0, n_clusters, (n,), device=device, generator=g)
+    noise = torch.randn((n, 3), dtype=torch.float64, device=device, generator=g) * 1.5
+    clu = comp == 2
+    xyz = torch.where(clu[:, None], centres[member] + noise, xyz)
+    return xyz.contiguous()
+
+
+def clustered_c3_small(frame=0):
+    """The clustered mixture at 1/16 of the points, the clusters and the
+    volume (box 60 x 60 x 90: x and y halved, z quartered), so the slab,
+    vapour and cluster densities are those of the full box."""
+    return clustered_c3(frame, n=4194304 // 16, n_clusters=2000 // 16,
+                        box=(60.0, 60.0, 90.0))
 
 
 def _clustered_in(rng, n, H, n_clusters):
