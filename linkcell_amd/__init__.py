@@ -254,7 +254,8 @@ class Workspace:
                   "sparse_tile_tests", "sparse_tiles", "sparse_evals",
                   "sparse_merges", "row_sources", "row_tile_tests", "row_tiles",
                   "row_merges", "row_rounds", "row_cells", "row_refines",
-                  "row_useful")
+                  "row_useful", "row_cyc_fill", "row_cyc_drain", "row_cyc_merge",
+                  "row_cyc_all")
 
     def last_stats(self) -> dict:
         n = len(self.STAT_NAMES)

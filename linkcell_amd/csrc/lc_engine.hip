@@ -578,6 +578,46 @@ __device__ __forceinline__ void insert_merge(Top<KC>& h, const long long* bd,
   }
 }
 
+// Merge up to 8 buffered candidates into the top-k (KC >= 8) with a fixed
+// network: sort the 8 (bitonic), take the element-wise min of the top
+// (ascending) and the batch reversed and padded with +inf (top[KC - 8 + t]
+// against batch[7 - t]); that min holds the KC smallest of both and is
+// bitonic, so one bitonic clean over all KC slots sorts it. Same kept set
+// and min_rej as insert_merge, at a cost that does not grow with the
+// number of entries.
+template <int KC>
+__device__ __forceinline__ void merge8(Top<KC>& h, const long long* bd,
+                                       const int* bj, int n, double& min_rej) {
+  long long b[8];
+  int bx[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const bool v = t < n;
+    b[t] = v ? bd[t * kWave] : kInfBits;
+    bx[t] = v ? bj[t * kWave] : kEmptyJ;
+  }
+  bitonic_sort<8>(b, bx);
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const long long x = h.d[KC - 8 + t], y = b[7 - t];
+    const int xj = h.j[KC - 8 + t], yj = bx[7 - t];
+    const bool ty = y < x;
+    h.d[KC - 8 + t] = ty ? y : x;
+    h.j[KC - 8 + t] = ty ? yj : xj;
+    const long long hi = ty ? x : y;
+    const int hij = ty ? xj : yj;
+    if (hij != kEmptyJ) min_rej = fmin(min_rej, as_f64(hi));
+  }
+#pragma unroll
+  for (int stride = KC >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+    for (int i = 0; i < KC; ++i) {
+      const int p = i ^ stride;
+      if (p > i) cswap(h.d[i], h.j[i], h.d[p], h.j[p], true);
+    }
+  }
+}
+
 // Finish a source with a full heap: the row is exact unless a rejected or
 // evicted d2 equals the k-th (a boundary tie, where the reference keeps
 // whichever index it met first). Equal d2 inside the row is ordered by
@@ -685,13 +725,9 @@ struct WalkArgs {
   const float* pred;
   const float* pred_tile;
   long long pred_tile_n;  // tiles beyond it are not capped
-  // capped row walk: a row some of whose sources the cap may have kept
-  // from the reference's stop sets its tile's flag here and emits nothing
-  // (null: such sources go to the exact-order walk instead)
-  unsigned char* retry;
+  // capped row walk: rows walked a second time without their cap are
+  // counted here (null: not counted)
   int* retry_count;
-  // retry pass: only the rows whose tile flag is set walk (null = all)
-  const unsigned char* only;
 };
 
 enum {
@@ -714,6 +750,10 @@ enum {
   kStatRowCells,    // wave-level candidate-cell visits of the row walk
   kStatRowRefines,  // row-level exact tile refinements
   kStatRowUseful,   // row-level tiles swept (the rest of the wave's rounds idle)
+  kStatRowCycFill,  // wave-level cycles listing tiles (stats build only)
+  kStatRowCycDrain, // wave-level cycles draining lists, merges included
+  kStatRowCycMerge, // wave-level cycles merging buffers into the heaps
+  kStatRowCycAll,   // wave-level cycles of the whole walk
   kStatCount = 24
 };
 
@@ -1326,12 +1366,8 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   const int nt_frame = A.tile_off[A.fcb[f + 1]] - tb;
   const int tf = (wg - row_wave0(A.tile_off, A.fcb, f)) * 4 + rw;
   if (__ballot(tf < nt_frame) == 0) return;
-  bool has_tile = tf < nt_frame;
+  const bool has_tile = tf < nt_frame;
   const int T = tb + (has_tile ? tf : 0);
-  if (A.only != nullptr) {
-    has_tile = has_tile && A.only[T] != 0;
-    if (__ballot(has_tile) == 0) return;
-  }
   const int g = A.tile_cell[T];
   const int o0 = A.off[g];
   const int cnt = A.off[g + 1] - o0;
@@ -1420,15 +1456,23 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   int* lmeta = &s_meta[wv][rw][0];
   unsigned st_tests = 0, st_tiles = 0, st_merges = 0, st_rounds = 0,
            st_cells = 0, st_exact = 0, st_useful = 0;
+  unsigned long long cyc_fill = 0, cyc_drain = 0, cyc_merge = 0;
+  const unsigned long long cyc0 = kStats ? __builtin_readcyclecounter() : 0;
 
   auto flush = [&]() {
+    const unsigned long long t0 = kStats ? __builtin_readcyclecounter() : 0;
     if (nbuf > 0) {
       if (kStats) st_merges += 1;
-      insert_merge<KC>(h, bd, bj, nbuf, min_rej);
+      if constexpr (KC >= 8) {
+        merge8<KC>(h, bd, bj, nbuf, min_rej);
+      } else {
+        insert_merge<KC>(h, bd, bj, nbuf, min_rej);
+      }
       nbuf = 0;
       thr = fmin(as_f64(h.d[KC - 1]), thr_cap);
     }
     rthr = red16_max(live ? thr : -inf);
+    if (kStats) cyc_merge += __builtin_readcyclecounter() - t0;
   };
 
   double cx = 0.0, cy = 0.0, cz = 0.0;
@@ -1471,11 +1515,17 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     }
   };
 
-  // opening: the row's own tile (its sources are the candidates, no
-  // shift), then the next tile of the home cell in Morton order
   const int nt_home = (cnt + kTile - 1) >> kTileShift;
   const int tl_nb = tl + 1 < nt_home ? tl + 1 : tl - 1;  // -1: none
   const double nan = __builtin_nan("");
+  // Walk, at most twice: a row some of whose sources the cap may have kept
+  // from the reference's stop (heap not full at a shell end with the cap
+  // below the stop bound) is walked again, whole and uncapped, by the same
+  // wave once every other row has finished.
+#pragma unroll 1
+  for (int attempt = 0; attempt < 2; ++attempt) {
+  // opening: the row's own tile (its sources are the candidates, no
+  // shift), then the next tile of the home cell in Morton order
   cx = valid ? px : nan;
   cy = valid ? py : nan;
   cz = valid ? pz : nan;
@@ -1497,6 +1547,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   // drain the list: tile i's candidates are loaded while tile i - 1 is
   // swept
   auto drain = [&]() {
+    const unsigned long long t0 = kStats ? __builtin_readcyclecounter() : 0;
     int i = 0;
     double nxq = nan, nyq = nan, nzq = nan;
     int njq = -1, ncode = 13;
@@ -1533,6 +1584,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       sweep();
     }
     lcount = 0;
+    if (kStats) cyc_drain += __builtin_readcyclecounter() - t0;
   };
 
 #pragma unroll 1
@@ -1545,6 +1597,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
 #pragma unroll 1
     while (true) {
       // fill the list while every filling row has room for 16 entries
+      const unsigned long long tf0 = kStats ? __builtin_readcyclecounter() : 0;
 #pragma unroll 1
       while (true) {
         const bool fill = more && lcount < kFill;
@@ -1681,6 +1734,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
           }
         }
       }
+      if (kStats) cyc_fill += __builtin_readcyclecounter() - tf0;
       if (__ballot(lcount > 0)) drain();
       if (!__ballot(more)) break;
     }
@@ -1706,6 +1760,21 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     rthr = red16_max(live ? thr : -inf);
     if (!__ballot(live)) break;
   }
+  const bool row_fail = ((__ballot(cap_fail) >> rbase) & 0xffffull) != 0;
+  if (!__ballot(row_fail)) break;
+  if (row_fail && li == 0 && A.retry_count != nullptr) atomicAdd(A.retry_count, 1);
+  // the failed rows start over without the cap; every other row is done
+  if (row_fail) {
+    top_init<KC>(h, k);
+    min_rej = inf;
+    thr_cap = inf;
+    live = valid;
+    thr = live ? inf : -inf;
+    done_reach = 0;
+    cap_fail = false;
+  }
+  rthr = red16_max(live ? thr : -inf);
+  }
   if (kStats) {
     stat_add(A.stats, kStatRowTileTests, st_tests);
     stat_add(A.stats, kStatRowMerges, st_merges);
@@ -1718,18 +1787,10 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     if (lane == 0) {
       stat_add(A.stats, kStatRowRounds, st_rounds);
       stat_add(A.stats, kStatRowCells, st_cells);
-    }
-  }
-  if (A.retry != nullptr) {
-    // a row with a cap failure is walked again, whole, without the cap by
-    // the retry pass; none of its sources emits or defers now
-    const unsigned long long fb = __ballot(cap_fail);
-    if (((fb >> rbase) & 0xffffull) != 0) {
-      if (li == 0 && has_tile) {
-        A.retry[T] = 1;
-        atomicAdd(A.retry_count, 1);
-      }
-      return;
+      stat_add(A.stats, kStatRowCycFill, cyc_fill);
+      stat_add(A.stats, kStatRowCycDrain, cyc_drain);
+      stat_add(A.stats, kStatRowCycMerge, cyc_merge);
+      stat_add(A.stats, kStatRowCycAll, __builtin_readcyclecounter() - cyc0);
     }
   }
   if (!valid) return;
@@ -2073,7 +2134,7 @@ void Engine::free_all() {
                 &cellslot_, &tmp_key_,  &cand_,
                 &scell_,    &byidx_,  &tbox_,   &cbox_,
                 &defer_,    &heap_d2_,  &heap_idx_, &flags_, &stats_,
-                &sched_,    &sched_cnt_, &tile_cell_, &retry_,
+                &sched_,    &sched_cnt_, &tile_cell_,
                 &pred_tile_, &labels_};
   for (Buf* b : all) {
     if (b->p != nullptr) (void)hipFree(b->p);
@@ -2324,7 +2385,10 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   wa.ncells = ctot;
   wa.tbox = as<Box>(tbox_.p);
   wa.cbox = as<Box>(cbox_.p);
+  // walk split: kRowWalk = rows everywhere; positive = a wave per dense
+  // cell, one lane per source elsewhere
   wa.dense_min = dense_min_for(frames);
+  const bool row_mode = wa.dense_min == kRowWalk;
   wa.tile_cell = as<int>(tile_cell_.p);
   wa.defer_list = as<int>(defer_.p);
   wa.defer_count = counters + 1;
@@ -2342,12 +2406,9 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   wa.pred = nullptr;
   wa.pred_tile = nullptr;
   wa.pred_tile_n = 0;
-  wa.retry = nullptr;
   wa.retry_count = nullptr;
-  wa.only = nullptr;
   const float* pred = nullptr;
-  if (sched_mode_ == 1 && pred_ != nullptr && pred_->trained() &&
-      wa.dense_min == kRowWalk) {
+  if (sched_mode_ == 1 && pred_ != nullptr && pred_->trained() && row_mode) {
     // learned caps of every source tile, from the counts and tile boxes
     // just built
     grow(pred_tile_, sizeof(float) * static_cast<size_t>(tcap));
@@ -2369,28 +2430,19 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   }
   wa.pred = pred;
   // capped row walk: rows whose cap may have hidden the reference's stop
-  // are walked again without it (second launch per heap bucket)
-  const bool capped =
-      (wa.pred != nullptr || wa.pred_tile != nullptr) && wa.dense_min == kRowWalk;
-  if (capped) {
-    const long long words = (tcap + 3) / 4;
-    grow(retry_, sizeof(unsigned) * static_cast<size_t>(words));
-    k_zero_u32<<<blocks_for(words, 256), 256, 0, st>>>(as<unsigned>(retry_.p),
-                                                      words);
-    wa.retry = as<unsigned char>(retry_.p);
-    wa.retry_count = counters + 3;
-  }
-  const float* depth = pred;
-  if (depth != nullptr && wa.dense_min != kRowWalk) {
+  // are walked again, uncapped, by their own wave; counted here
+  wa.retry_count = counters + 3;
+  // depth order of the lane-per-source walk from the per-cell prediction
+  if (pred != nullptr && !row_mode) {
     grow(sched_, sizeof(int) * static_cast<size_t>(ntot));
     grow(sched_cnt_, sizeof(int) * 2 * kSchedBuckets);
     int* cnt = as<int>(sched_cnt_.p);
     k_zero_u32<<<1, 64, 0, st>>>(as<unsigned>(sched_cnt_.p), 2 * kSchedBuckets);
     k_sched<false><<<blocks_for(ntot, 256), 256, 0, st>>>(
-        dfr, dfcb, F, as<int>(scell_.p), as<int>(off_.p), ctot, depth,
+        dfr, dfcb, F, as<int>(scell_.p), as<int>(off_.p), ctot, pred,
         wa.dense_min, cnt, cnt + kSchedBuckets, as<int>(sched_.p));
     k_sched<true><<<blocks_for(ntot, 256), 256, 0, st>>>(
-        dfr, dfcb, F, as<int>(scell_.p), as<int>(off_.p), ctot, depth,
+        dfr, dfcb, F, as<int>(scell_.p), as<int>(off_.p), ctot, pred,
         wa.dense_min, cnt, cnt + kSchedBuckets, as<int>(sched_.p));
     check(hipGetLastError(), "schedule launch");
     wa.sched = as<int>(sched_.p);
@@ -2399,16 +2451,9 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   const unsigned sparse_blocks = blocks_for(ntot, 256);
   // row walk: one wave per 4 source tiles, frames on wave boundaries
   const unsigned row_blocks = blocks_for(tcap / 4 + F + 1, 4);
-  WalkArgs wr = wa;  // retry pass: uncapped, flagged rows only
-  wr.pred = nullptr;
-  wr.pred_tile = nullptr;
-  wr.pred_tile_n = 0;
-  wr.retry = nullptr;
-  wr.retry_count = nullptr;
-  wr.only = wa.retry;
   if (timing) check(hipEventRecord(ev_[8], st), "event record");
 #define LC_WALK(KC)                                                           \
-  if (any_fast[KC] && wa.dense_min == kRowWalk) {                             \
+  if (any_fast[KC] && row_mode) {                                             \
     if (wa.stats != nullptr) {                                                \
       k_walk_rows<KC, true><<<row_blocks, 256, 0, st>>>(wa);                  \
     } else {                                                                  \
@@ -2428,21 +2473,6 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   LC_WALK(32)
 #undef LC_WALK
   if (timing) check(hipEventRecord(ev_[9], st), "event record");
-  // retry pass of the capped row walk: the flagged rows, uncapped
-#define LC_RETRY(KC)                                                          \
-  if (capped && any_fast[KC]) {                                               \
-    if (wa.stats != nullptr) {                                                \
-      k_walk_rows<KC, true><<<row_blocks, 256, 0, st>>>(wr);                  \
-    } else {                                                                  \
-      k_walk_rows<KC, false><<<row_blocks, 256, 0, st>>>(wr);                 \
-    }                                                                         \
-    check(hipGetLastError(), "retry walk launch");                            \
-  }
-  LC_RETRY(4)
-  LC_RETRY(8)
-  LC_RETRY(16)
-  LC_RETRY(32)
-#undef LC_RETRY
   // frames outside every register bucket go straight to the exact walk
   bool any_exact_frame = false;
   for (int f = 0; f < F; ++f) {

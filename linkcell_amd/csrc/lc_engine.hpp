@@ -76,11 +76,11 @@ struct FrameSpec {
 struct Timings {
   float fold = 0, scan = 0, scatter = 0, sort = 0, walk = 0, exact = 0,
         total = 0;
-  // parts of walk: schedule set-up (learned caps, depth order), the first
-  // (capped) pass of every walk kernel, the retry pass of the capped rows
+  // parts of walk: schedule set-up (learned caps, depth order), the walk
+  // kernels, and what follows them before the exact walk
   float setup = 0, pass1 = 0, retry = 0;
   int deferred = 0;  // sources finished by the exact-order walk
-  int retried = 0;   // rows walked again without their predicted cap
+  int retried = 0;   // rows walked a second time without their predicted cap
 };
 
 enum class WalkMode : int {
@@ -171,7 +171,7 @@ class Engine {
   Buf frames_, fpt_, fcb_, count_, status_, counters_, off_, tile_off_,
       item_off_, items_, cellslot_, tmp_key_, cand_, scell_,
       byidx_, tbox_, cbox_, defer_, heap_d2_, heap_idx_, flags_, stats_,
-      sched_, sched_cnt_, tile_cell_, retry_;
+      sched_, sched_cnt_, tile_cell_;
   int sched_mode_ = 0;
   const float* given_pred_ = nullptr;
   long long given_pred_n_ = 0;
