@@ -754,6 +754,7 @@ enum {
   kStatRowCycDrain, // wave-level cycles draining lists, merges included
   kStatRowCycMerge, // wave-level cycles merging buffers into the heaps
   kStatRowCycAll,   // wave-level cycles of the whole walk
+  kStatRowFlushes,  // wave-level buffer merges
   kStatCount = 24
 };
 
@@ -1457,10 +1458,12 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   unsigned st_tests = 0, st_tiles = 0, st_merges = 0, st_rounds = 0,
            st_cells = 0, st_exact = 0, st_useful = 0;
   unsigned long long cyc_fill = 0, cyc_drain = 0, cyc_merge = 0;
+  unsigned st_flush = 0;
   const unsigned long long cyc0 = kStats ? __builtin_readcyclecounter() : 0;
 
   auto flush = [&]() {
     const unsigned long long t0 = kStats ? __builtin_readcyclecounter() : 0;
+    if (kStats) st_flush += 1;
     if (nbuf > 0) {
       if (kStats) st_merges += 1;
       if constexpr (KC >= 8) {
@@ -1492,26 +1495,29 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       nbuf += 1;
     }
   };
-  auto quad = [&](auto B) {
+  auto pair = [&](auto B) {
     constexpr int b0 = decltype(B)::value;
     step(std::integral_constant<int, b0>{});
     step(std::integral_constant<int, b0 + 1>{});
-    step(std::integral_constant<int, b0 + 2>{});
-    step(std::integral_constant<int, b0 + 3>{});
   };
   // the 16 candidates held in cx..cj (one per lane of each row) against
-  // every source of the row; one merge site checked after each quad
+  // every source of the row; one merge site, checked after every two
+  // candidates, so buffers fill to BC - 1 or BC before a merge
   auto sweep = [&]() {
     if (kStats) st_rounds += 1;
 #pragma unroll 1
-    for (int qd = 0; qd < 4; ++qd) {
-      switch (qd) {
-        case 0: quad(std::integral_constant<int, 0>{}); break;
-        case 1: quad(std::integral_constant<int, 4>{}); break;
-        case 2: quad(std::integral_constant<int, 8>{}); break;
-        default: quad(std::integral_constant<int, 12>{}); break;
+    for (int pr = 0; pr < 8; ++pr) {
+      switch (pr) {
+        case 0: pair(std::integral_constant<int, 0>{}); break;
+        case 1: pair(std::integral_constant<int, 2>{}); break;
+        case 2: pair(std::integral_constant<int, 4>{}); break;
+        case 3: pair(std::integral_constant<int, 6>{}); break;
+        case 4: pair(std::integral_constant<int, 8>{}); break;
+        case 5: pair(std::integral_constant<int, 10>{}); break;
+        case 6: pair(std::integral_constant<int, 12>{}); break;
+        default: pair(std::integral_constant<int, 14>{}); break;
       }
-      if (__ballot(nbuf > BC - 4)) flush();
+      if (__ballot(nbuf > BC - 2)) flush();
     }
   };
 
@@ -1791,6 +1797,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       stat_add(A.stats, kStatRowCycDrain, cyc_drain);
       stat_add(A.stats, kStatRowCycMerge, cyc_merge);
       stat_add(A.stats, kStatRowCycAll, __builtin_readcyclecounter() - cyc0);
+      stat_add(A.stats, kStatRowFlushes, st_flush);
     }
   }
   if (!valid) return;
