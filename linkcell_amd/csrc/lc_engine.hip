@@ -1445,8 +1445,8 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     const double q = static_cast<double>(A.pred_tile[T]);
     thr_cap = fmin(thr_cap, q * q);
   }
-  double thr = thr_cap;
   bool live = valid;
+  double thr = live ? thr_cap : -inf;  // -inf: finished (never accepts)
   bool cap_fail = false;
   double rthr = red16_max(live ? thr : -inf);  // -inf: no live source
   int nbuf = 0;
@@ -1472,7 +1472,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
         insert_merge<KC>(h, bd, bj, nbuf, min_rej);
       }
       nbuf = 0;
-      thr = fmin(as_f64(h.d[KC - 1]), thr_cap);
+      thr = live ? fmin(as_f64(h.d[KC - 1]), thr_cap) : -inf;
     }
     rthr = red16_max(live ? thr : -inf);
     if (kStats) cyc_merge += __builtin_readcyclecounter() - t0;
@@ -1480,42 +1480,53 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
 
   double cx = 0.0, cy = 0.0, cz = 0.0;
   int cj = -1;
-  auto step = [&](auto I) {
+  // One candidate (lane I of each row) against every source of the row.
+  // Empty candidate slots carry NaN coordinates and finished sources a
+  // threshold of -inf: neither is ever accepted. The index is broadcast only
+  // when some lane of the wave accepts (outside the per-lane branch, where
+  // every lane is active for the DPP read). SELF: the candidates are the
+  // row's own sources, and the source itself is skipped; no other tile of
+  // the walk holds it (its own tile is swept only in the opening).
+  auto step = [&](auto I, auto selftag) {
     constexpr int ii = decltype(I)::value;
+    constexpr bool SELF = decltype(selftag)::value;
     const double qx = row_bcast<ii>(cx);
     const double qy = row_bcast<ii>(cy);
     const double qz = row_bcast<ii>(cz);
-    const int qj = row_bcast<ii>(cj);
     const double ddx = qx - px, ddy = qy - py, ddz = qz - pz;
     const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-    // empty candidate slots carry NaN coordinates: never accepted
-    if (live && d2 <= thr && qj != me) {
-      bd[nbuf * kWave] = __double_as_longlong(d2);
-      bj[nbuf * kWave] = qj;
-      nbuf += 1;
+    bool acc = d2 <= thr;
+    if (SELF) acc = acc && row_bcast<ii>(cj) != me;
+    if (__ballot(acc)) {
+      const int qj = row_bcast<ii>(cj);
+      if (acc) {
+        bd[nbuf * kWave] = __double_as_longlong(d2);
+        bj[nbuf * kWave] = qj;
+        nbuf += 1;
+      }
     }
   };
-  auto pair = [&](auto B) {
+  auto pair = [&](auto B, auto selftag) {
     constexpr int b0 = decltype(B)::value;
-    step(std::integral_constant<int, b0>{});
-    step(std::integral_constant<int, b0 + 1>{});
+    step(std::integral_constant<int, b0>{}, selftag);
+    step(std::integral_constant<int, b0 + 1>{}, selftag);
   };
   // the 16 candidates held in cx..cj (one per lane of each row) against
   // every source of the row; one merge site, checked after every two
   // candidates, so buffers fill to BC - 1 or BC before a merge
-  auto sweep = [&]() {
+  auto sweep = [&](auto selftag) {
     if (kStats) st_rounds += 1;
 #pragma unroll 1
     for (int pr = 0; pr < 8; ++pr) {
       switch (pr) {
-        case 0: pair(std::integral_constant<int, 0>{}); break;
-        case 1: pair(std::integral_constant<int, 2>{}); break;
-        case 2: pair(std::integral_constant<int, 4>{}); break;
-        case 3: pair(std::integral_constant<int, 6>{}); break;
-        case 4: pair(std::integral_constant<int, 8>{}); break;
-        case 5: pair(std::integral_constant<int, 10>{}); break;
-        case 6: pair(std::integral_constant<int, 12>{}); break;
-        default: pair(std::integral_constant<int, 14>{}); break;
+        case 0: pair(std::integral_constant<int, 0>{}, selftag); break;
+        case 1: pair(std::integral_constant<int, 2>{}, selftag); break;
+        case 2: pair(std::integral_constant<int, 4>{}, selftag); break;
+        case 3: pair(std::integral_constant<int, 6>{}, selftag); break;
+        case 4: pair(std::integral_constant<int, 8>{}, selftag); break;
+        case 5: pair(std::integral_constant<int, 10>{}, selftag); break;
+        case 6: pair(std::integral_constant<int, 12>{}, selftag); break;
+        default: pair(std::integral_constant<int, 14>{}, selftag); break;
       }
       if (__ballot(nbuf > BC - 2)) flush();
     }
@@ -1536,7 +1547,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   cy = valid ? py : nan;
   cz = valid ? pz : nan;
   cj = valid ? me : -1;
-  sweep();
+  sweep(std::true_type{});
   cx = cy = cz = nan;
   cj = -1;
   if (has_tile && tl_nb >= 0 && tl_nb * kTile + li < cnt) {
@@ -1546,7 +1557,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     cz = q.z;
     cj = static_cast<int>(q.id);
   }
-  if (__ballot(tl_nb >= 0 && has_tile)) sweep();
+  if (__ballot(tl_nb >= 0 && has_tile)) sweep(std::false_type{});
   if (__ballot(nbuf > 0)) flush();
 
   int lcount = 0;  // entries in this row's list
@@ -1587,7 +1598,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       i += 1;
       fetch(i);
       if (kStats) st_tiles += 1;
-      sweep();
+      sweep(std::false_type{});
     }
     lcount = 0;
     if (kStats) cyc_drain += __builtin_readcyclecounter() - t0;
@@ -1763,6 +1774,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     }
     // beyond r_safe the stencil wraps onto itself: exact walk
     if (live && r + 1 > d.r_safe) live = false;
+    if (!live) thr = -inf;
     rthr = red16_max(live ? thr : -inf);
     if (!__ballot(live)) break;
   }
