@@ -169,41 +169,83 @@ def copy_calibration(mib):
 def bench_batch(args):
     """Uneven batch of 256 frames (n from 1e3 to 2e5, orthorhombic and
     triclinic, uniform and clustered, k in {4, 12, 32}) in one pipeline,
-    captured once into a hipGraph and replayed."""
+    captured once into a hipGraph and replayed: the plain schedule, then
+    the learned one (trained on four other 256-frame batches of the same
+    generator), checked bitwise against the plain outputs."""
     import torch
     import linkcell_amd as lc
     from linkcell_amd import workloads as wl
-    frames = wl.batched_c4(256)
     dev = torch.device("cuda:0")
-    keep, recs = [], []
-    ntot = 0
-    for fr in frames:
-        n, k = len(fr["xyz"]), fr["k"]
-        x = torch.from_numpy(np.ascontiguousarray(fr["xyz"])).to(dev)
-        nn = torch.empty((n, k), dtype=torch.int32, device=dev)
-        d2 = torch.empty((n, k), dtype=torch.float64, device=dev)
-        keep += [x, nn, d2]
-        c = np.asarray(fr["cell"], dtype=np.float64)
-        recs.append(dict(xyz=x.data_ptr(), n=n, cell=list(c.reshape(9)) + [0.0, 0.0, 0.0],
-                         k=k, cell_hint=fr["hint"], out_nn=nn.data_ptr(),
-                         out_d2=d2.data_ptr()))
-        ntot += n
+
+    def device_batch(frames):
+        keep, recs = [], []
+        for fr in frames:
+            n, k = len(fr["xyz"]), fr["k"]
+            x = torch.from_numpy(np.ascontiguousarray(fr["xyz"])).to(dev)
+            nn = torch.empty((n, k), dtype=torch.int32, device=dev)
+            d2 = torch.empty((n, k), dtype=torch.float64, device=dev)
+            keep += [x, nn, d2]
+            c = np.asarray(fr["cell"], dtype=np.float64)
+            recs.append(dict(xyz=x.data_ptr(), n=n, cell=list(c.reshape(9)) + [0.0, 0.0, 0.0],
+                             k=k, cell_hint=fr["hint"], out_nn=nn.data_ptr(),
+                             out_d2=d2.data_ptr()))
+        torch.cuda.synchronize()
+        return keep, recs
+
+    frames = wl.batched_c4(256)
+    ntot = sum(len(fr["xyz"]) for fr in frames)
+    keep, recs = device_batch(frames)
     ws = lc.Workspace()
-    torch.cuda.synchronize()
     best, tune_ms = ws.autotune(recs)
     log(f"bench: batch of {len(frames)} frames, {ntot} points; tuned split {best} "
         f"({tune_ms:.3f} ms)")
-    ws.graph_capture(recs)
-    for _ in range(args.warmup):
-        ws.graph_launch(wait=False)
-    ws.wait()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ws.graph_launch(wait=False)
-    ws.wait()
-    torch.cuda.synchronize()
-    ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
+
+    def timed_graph(tag):
+        ws.graph_capture(recs)
+        for _ in range(args.warmup):
+            ws.graph_launch(wait=False)
+        ws.wait()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ws.graph_launch(wait=False)
+        ws.wait()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        ws.graph_reset()
+        log(f"bench: {tag}: {ms:.3f} ms per replay")
+        return ms
+
+    ws.set_schedule(0)
+    ms_plain = timed_graph("plain graph")
+    ref = [t.clone() for t in keep[1::3]] + [t.clone() for t in keep[2::3]]
+    learned = None
+    sched = "plain"
+    ms_step = ms_plain
+    if args.schedule == "learned" and best == -1:
+        t0 = time.perf_counter()
+        ws.sched_clear()
+        for b in range(4):
+            tk, tr = device_batch(wl.batched_c4(256, seed0=10000 + 256 * b))
+            ws.knearest_batch(tr)
+            ws.sched_collect()
+            del tk
+        loss, margin = ws.sched_train(epochs=args.epochs, batch=4096, lr=2e-3, seed=1,
+                                      quantile=args.quantile)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ws.set_schedule(1)
+        ms_l = timed_graph("learned graph")
+        got = keep[1::3] + keep[2::3]
+        same = all(bool(torch.equal(g.view(torch.int64) if g.dtype == torch.float64 else g,
+                                    r.view(torch.int64) if r.dtype == torch.float64 else r))
+                   for g, r in zip(got, ref))
+        if not same:
+            raise SystemExit("bench: the learned walk differs from the plain walk")
+        learned = {"ms_per_step": ms_l, "train_batches": 4, "samples": ws.sched_samples(),
+                   "loss": loss, "margin_log2": margin, "train_s": t1 - t0,
+                   "bitwise_equal_to_plain": same}
+        ms_step, sched = ms_l, "learned"
     out = {
         "metric": "source-queries/sec, 256 uneven frames in one captured pipeline",
         "value": ntot / (ms_step * 1e-3),
@@ -213,8 +255,12 @@ def bench_batch(args):
         "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded generator, see linkcell_amd/workloads.py)",
         "config": {"workload": "uneven-batch-256-frames", "points": ntot,
-                   "k": [4, 12, 32], "split": best, "graph": True},
+                   "k": [4, 12, 32], "split": best, "graph": True, "schedule": sched},
+        "plain": {"value": ntot / (ms_plain * 1e-3), "ms_per_step": ms_plain},
     }
+    if learned is not None:
+        out["learned"] = learned
+        out["learned_over_plain"] = ms_plain / ms_step
     print(json.dumps(out), flush=True)
 
 

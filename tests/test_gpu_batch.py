@@ -160,6 +160,33 @@ def test_graph_replay_follows_refilled_inputs(ws, c4):
         ws.graph_reset()
 
 
+def test_learned_schedule_in_captured_graph(ws, c4, c4_oracle):
+    """The learned schedule inside a captured batch: train the predictor on
+    other uneven batches (labels from their plain walk), capture this batch
+    under the learned schedule, replay it with eager work between replays;
+    every frame stays bit-identical to its own oracle call."""
+    train = wl.batched_c4(12, seed0=4700)
+    ws.set_schedule(0)
+    ws.sched_clear()
+    tb = DeviceBatch(train)
+    ws.knearest_batch(tb.recs)
+    assert ws.sched_collect() == ws.tile_count()
+    loss, margin = ws.sched_train(epochs=20, batch=1024, lr=2e-3, seed=3, quantile=0.98)
+    assert np.isfinite(loss) and np.isfinite(margin)
+    b = DeviceBatch(c4[:10])
+    ws.set_schedule(1)
+    try:
+        ws.graph_capture(b.recs)
+        for rep in range(3):
+            b.clear()
+            ws.graph_launch(wait=True)
+            assert_frames_equal(b.results(), c4_oracle[:10], f"learned replay {rep}")
+            ws.knearest_batch(b.recs[:2])
+    finally:
+        ws.graph_reset()
+        ws.set_schedule(0)
+
+
 def test_graph_launch_without_capture_raises(ws):
     import linkcell_amd as lc
     ws.graph_reset()
