@@ -65,6 +65,7 @@ constexpr int kChunkShift = 6;  // sources per dense work item (one wave)
 constexpr int kPad = 16;        // slack elements after SoA arrays (group reads)
 constexpr int kNeverDense = 1 << 30;  // dense_min that keeps every cell sparse
 constexpr int kRowWalk = -1;          // split value selecting the row walk
+constexpr int kExactWaveMaxK = 2 * kWave;  // wave-per-source exact walk: k up to two slots per lane
 
 // Cell-ordered candidate record: folded coordinates and the point's index
 // within its frame (64-bit, so a record is 32 bytes: one s_load_dwordx8
@@ -1320,13 +1321,14 @@ __device__ __forceinline__ int row_wave0(const int* tile_off,
 #ifndef LC_ROW_FILL
 #define LC_ROW_FILL 16
 #endif
-// Resident 256-thread blocks per CU the register budget is cut for: 4
-// waves per SIMD up to KC = 16, 3 for KC = 32 (measured on the clustered
-// and FCC workloads; the walk is bound by memory latency, so occupancy
-// pays more than the few spills it costs).
+// Resident 256-thread blocks per CU the register budget is cut for: 3 (3
+// waves per SIMD, 168 VGPRs) for every heap bucket. Measured on the
+// clustered 4M (k = 16), the uneven batch and the FCC 8M (k = 32) workloads
+// against 2 and 4: with 4 the spills of the larger heaps cost more than the
+// extra wave hides, with 2 too little latency is hidden.
 template <int KC>
 struct RowOcc {
-  static constexpr int kMinBlocks = KC >= 32 ? 3 : 4;
+  static constexpr int kMinBlocks = 3;
 };
 #ifdef LC_ROW_MINB
 #define LC_ROW_BLOCKS(KC) LC_ROW_MINB
@@ -1941,8 +1943,9 @@ __global__ __launch_bounds__(64) void k_walk_exact(
     const int g = scell[s];
     const int f = find_frame(fcb, F, g);
     const FrameDesc& d = fr[f];
-    const CellGeom gm = geom_of(d);
     const int k = d.k;
+    if (k <= kExactWaveMaxK) continue;  // k_walk_exact_wave's
+    const CellGeom gm = geom_of(d);
     const int nx = d.nx, ny = d.ny, nz = d.nz;
     const long long cb = d.cell_base;
     const int c = static_cast<int>(g - cb);
@@ -1978,10 +1981,13 @@ __global__ __launch_bounds__(64) void k_walk_exact(
               if (j == me) continue;
               const double d2 = dist2_shifted(px, py, pz, cq.x, cq.y, cq.z,
                                               shx, shy, shz);
-              // KHeap::push. With a full heap, d2 >= the worst slot can
-              // change nothing: an index already present holds at most
-              // that value, and the replacement needs d2 < worst.
-              if (n == k && !(d2 < hd[worst * st])) continue;
+              // KHeap::push. With a full heap and a worst slot that is a
+              // number, d2 >= worst can change nothing: an index already
+              // present holds at most that value, and the replacement
+              // needs d2 < worst. (A NaN worst, kept by the first-argmax
+              // scan when slot 0 is NaN, bounds nothing: push fully.)
+              const double wv = hd[worst * st];
+              if (n == k && wv == wv && !(d2 < wv)) continue;
               bool found = false;
               for (int u = 0; u < n; ++u) {
                 if (hj[u * st] == j) {
@@ -1995,7 +2001,7 @@ __global__ __launch_bounds__(64) void k_walk_exact(
                   hd[n * st] = d2;
                   hj[n * st] = j;
                   n += 1;
-                } else {
+                } else if (d2 < hd[worst * st]) {
                   hd[worst * st] = d2;
                   hj[worst * st] = j;
                 }
@@ -2048,6 +2054,199 @@ __global__ __launch_bounds__(64) void k_walk_exact(
       if (d.out_d2) d.out_d2[row + t] = real ? hd[t * st] : nan_value();
     }
     if (d.out_reach) d.out_reach[me] = final_reach;
+  }
+}
+
+// Wave-level KHeap::worst over the heap slots u < n (slot u = lane for the
+// first register pair, kWave + lane for the second): the sequential scan of
+// src/knearest.rs:135-144 (w = 0; w = u whenever d[u] > d[w]) keeps slot 0
+// when it holds NaN and otherwise never moves onto a NaN, so it is the first
+// argmax of the non-NaN values unless slot 0 is NaN.
+__device__ __forceinline__ void wave_worst(double d0, double d1, int n,
+                                           int lane, double& wv, int& ws) {
+  const double first = __shfl(d0, 0, kWave);
+  if (first != first) {
+    wv = first;
+    ws = 0;
+    return;
+  }
+  double v = -__builtin_inf();
+  int u = 1 << 30;
+  if (lane < n && d0 == d0) {
+    v = d0;
+    u = lane;
+  }
+  if (kWave + lane < n && d1 == d1 && (d1 > v || u == (1 << 30))) {
+    v = d1;
+    u = kWave + lane;
+  }
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) {
+    const double ov = __shfl_xor(v, m, kWave);
+    const int ou = __shfl_xor(u, m, kWave);
+    if (ov > v || (ov == v && ou < u)) {
+      v = ov;
+      u = ou;
+    }
+  }
+  wv = v;
+  ws = u;
+}
+
+// Exact-order replay of the reference walk and KHeap for deferred sources,
+// one wave per source (k <= kExactWaveMaxK): the lexicographic stencil
+// (src/knearest.rs:435-465), each cell in decreasing index order (through
+// byidx), the cell's candidates loaded and measured 64 at a time. A push
+// can only change the heap while it is not full or when d2 < worst (an
+// index already present holds at most worst, and a replacement needs d2 <
+// worst; a NaN worst bounds nothing), so only those candidates are pushed,
+// one after another in visit order: the heap sees the reference's sequence
+// of effective pushes. The
+// heap is spread over the wave, two slots per lane; KHeap::push, worst and
+// finish (src/knearest.rs:106-158) run as wave-wide ballots and reductions.
+__global__ __launch_bounds__(64) void k_walk_exact_wave(
+    const FrameDesc* __restrict__ fr, const long long* __restrict__ fcb,
+    int F, const Cand* __restrict__ cand, const int* __restrict__ scell,
+    const int* __restrict__ off, const int* __restrict__ byidx,
+    const int* __restrict__ defer_list, const int* __restrict__ defer_count) {
+  __shared__ long long s_key[kExactWaveMaxK];
+  __shared__ int s_j[kExactWaveMaxK];
+  const int lane = threadIdx.x;
+  const int nq = *defer_count;
+  for (int qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+    const int s = uni(defer_list[qi]);
+    const int g = uni(scell[s]);
+    const int f = uni(find_frame(fcb, F, g));
+    const FrameDesc& d = fr[f];
+    const int k = d.k;
+    if (k > kExactWaveMaxK) continue;  // the lane-per-source replay's
+    const CellGeom gm = geom_of(d);
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const long long cb = d.cell_base;
+    const int c = static_cast<int>(g - cb);
+    const int hx = c % nx, hy = (c / nx) % ny, hz = c / (nx * ny);
+    const Cand mc = cand[s];
+    const double px = mc.x, py = mc.y, pz = mc.z;
+    const int me = static_cast<int>(mc.id);
+    double hd0 = __builtin_inf(), hd1 = __builtin_inf();
+    int hj0 = -1, hj1 = -1;
+    int n = 0;            // filled slots (uniform)
+    double worst = 0.0;   // KHeap::worst value when n == k (uniform)
+    int wslot = 0;        // its slot
+    int final_reach = d.max_reach;
+    for (int reach = 1; reach <= d.max_reach; ++reach) {
+      for (int dx = -reach; dx <= reach; ++dx) {
+        for (int dy = -reach; dy <= reach; ++dy) {
+          for (int dz = -reach; dz <= reach; ++dz) {
+            const bool shell = reach == 1 || abs(dx) == reach ||
+                               abs(dy) == reach || abs(dz) == reach;
+            if (!shell && reach > 1) continue;
+            const int jx = hx + dx, jy = hy + dy, jz = hz + dz;
+            const long long cq = (static_cast<long long>(rem_euclid(jz, nz)) * ny +
+                                  rem_euclid(jy, ny)) * nx +
+                                 rem_euclid(jx, nx);
+            double shx, shy, shz;
+            lattice_shift(gm, div_euclid(jx, nx), div_euclid(jy, ny),
+                          div_euclid(jz, nz), shx, shy, shz);
+            const int a0 = uni(off[cb + cq]);
+            const int a1 = uni(off[cb + cq + 1]);
+            for (int top = a1 - 1; top >= a0; top -= kWave) {
+              const int t = top - lane;  // decreasing index order
+              const bool valid = t >= a0;
+              int j = -1;
+              double d2 = __builtin_inf();
+              if (valid) {
+                const Cand q = cand[byidx[t]];
+                j = static_cast<int>(q.id);
+                d2 = dist2_shifted(px, py, pz, q.x, q.y, q.z, shx, shy, shz);
+              }
+              // a NaN worst bounds nothing (see k_walk_exact)
+              const bool need =
+                  valid && j != me && (n < k || d2 < worst || worst != worst);
+              unsigned long long m = __ballot(need);
+              while (m != 0) {
+                const int src = __ffsll(static_cast<long long>(m)) - 1;
+                m &= m - 1;
+                const int cj = __shfl(j, src, kWave);
+                const double cd = __shfl(d2, src, kWave);
+                if (n == k && worst == worst && !(cd < worst)) continue;  // no effect
+                const bool in0 = lane < n && hj0 == cj;
+                const bool in1 = kWave + lane < n && hj1 == cj;
+                if (__ballot(in0 || in1) != 0) {
+                  if (in0 && cd < hd0) hd0 = cd;
+                  if (in1 && cd < hd1) hd1 = cd;
+                } else if (n < k) {
+                  if (n < kWave) {
+                    if (lane == n) {
+                      hd0 = cd;
+                      hj0 = cj;
+                    }
+                  } else if (lane == n - kWave) {
+                    hd1 = cd;
+                    hj1 = cj;
+                  }
+                  n += 1;
+                } else if (!(cd < worst)) {
+                  // full, absent, not below worst (a NaN worst): no change
+                } else if (wslot < kWave) {
+                  if (lane == wslot) {
+                    hd0 = cd;
+                    hj0 = cj;
+                  }
+                } else if (lane == wslot - kWave) {
+                  hd1 = cd;
+                  hj1 = cj;
+                }
+                if (n == k) wave_worst(hd0, hd1, n, lane, worst, wslot);
+              }
+            }
+          }
+        }
+      }
+      if (n >= k) {
+        double w;
+        int ws;
+        wave_worst(hd0, hd1, n, lane, w, ws);
+        const double bound = static_cast<double>(reach) * d.cell_min;
+        if (w <= bound * bound) {
+          final_reach = reach;
+          break;
+        }
+      }
+    }
+    // KHeap::finish: order by (total_cmp(d2), index); indices are distinct
+    if (lane < n) {
+      s_key[lane] = total_key(hd0);
+      s_j[lane] = hj0;
+    }
+    if (kWave + lane < n) {
+      s_key[kWave + lane] = total_key(hd1);
+      s_j[kWave + lane] = hj1;
+    }
+    __syncthreads();  // one wave per block
+    int r0 = 0, r1 = 0;
+    const long long k0 = total_key(hd0), k1 = total_key(hd1);
+    for (int u = 0; u < n; ++u) {
+      const long long ku = s_key[u];
+      const int ju = s_j[u];
+      r0 += (ku < k0 || (ku == k0 && ju < hj0)) ? 1 : 0;
+      r1 += (ku < k1 || (ku == k1 && ju < hj1)) ? 1 : 0;
+    }
+    const size_t row = static_cast<size_t>(me) * k;
+    if (lane < n) {
+      d.out_nn[row + r0] = hj0;
+      if (d.out_d2) d.out_d2[row + r0] = hd0;
+    }
+    if (kWave + lane < n) {
+      d.out_nn[row + r1] = hj1;
+      if (d.out_d2) d.out_d2[row + r1] = hd1;
+    }
+    for (int t = n + lane; t < k; t += kWave) {
+      d.out_nn[row + t] = -1;
+      if (d.out_d2) d.out_d2[row + t] = nan_value();
+    }
+    if (lane == 0 && d.out_reach) d.out_reach[me] = final_reach;
+    __syncthreads();  // the next source reuses s_key / s_j
   }
 }
 
@@ -2504,11 +2703,16 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
     check(hipGetLastError(), "route launch");
   }
   if (timing) check(hipEventRecord(ev_[5], st), "event record");
-  k_walk_exact<<<static_cast<unsigned>(exact_threads / 64), 64, 0, st>>>(
+  k_walk_exact_wave<<<2048, 64, 0, st>>>(
       dfr, dfcb, F, as<Cand>(cand_.p), as<int>(scell_.p), as<int>(off_.p),
-      as<int>(byidx_.p),
-      as<int>(defer_.p), counters + 1, as<double>(heap_d2_.p),
-      as<int>(heap_idx_.p));
+      as<int>(byidx_.p), as<int>(defer_.p), counters + 1);
+  if (kmax_exact > kExactWaveMaxK) {
+    k_walk_exact<<<static_cast<unsigned>(exact_threads / 64), 64, 0, st>>>(
+        dfr, dfcb, F, as<Cand>(cand_.p), as<int>(scell_.p), as<int>(off_.p),
+        as<int>(byidx_.p),
+        as<int>(defer_.p), counters + 1, as<double>(heap_d2_.p),
+        as<int>(heap_idx_.p));
+  }
   check(hipGetLastError(), "exact walk launch");
   if (timing) {
     check(hipEventRecord(ev_[6], st), "event record");

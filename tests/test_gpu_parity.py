@@ -224,3 +224,29 @@ def test_device_errors(ws):
         run_device(ws, np.zeros((4, 3)), [0.0, 1.0, 1.0], 2)
     with pytest.raises(lc.Error, match="too many cells"):
         run_device(ws, np.zeros((4, 3)), [1e4, 1e4, 1e4], 2, None, 1e-3)
+
+
+@pytest.mark.parametrize("k", [3, 16, 40])
+def test_non_finite_coordinates_follow_the_oracle(ws, k):
+    # A frame with NaN / +-inf coordinates: every source takes the exact-order
+    # walk (src/cell.rs:176-207 folds NaN into cell 0, src/knearest.rs:106-158
+    # keeps NaN distances by the same rules). The kept set is the oracle's;
+    # finite entries match in order and bits. A NaN distance's sign is not
+    # the reference's to fix (IEEE leaves NaN propagation to the platform:
+    # x86's invalid-operation NaN is negative, a - b on the GPU is a + (-b),
+    # and with two NaN operands x86 keeps the one the compiler put in the
+    # destination register), and f64::total_cmp sorts negative NaN first and
+    # positive NaN last: NaN entries match as a set of indices.
+    rng = np.random.default_rng(12)
+    xyz = rng.random((400, 3)) * 9.0
+    xyz[5, 0] = np.nan
+    xyz[17, 1] = np.inf
+    xyz[33, 2] = -np.inf
+    xyz[90] = np.nan
+    nn, d2 = run_device(ws, xyz, [9.0, 9.5, 10.0], k, None, 3.0)
+    on, od = oracle(xyz, [9.0, 9.5, 10.0], k, None, 3.0)
+    for i in range(len(nn)):
+        gf, of = ~np.isnan(d2[i]), ~np.isnan(od[i])
+        assert np.array_equal(nn[i][gf], on[i][of]), i
+        assert same_bits(d2[i][gf], od[i][of]), i
+        assert sorted(nn[i][~gf].tolist()) == sorted(on[i][~of].tolist()), i
