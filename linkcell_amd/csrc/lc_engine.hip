@@ -1318,8 +1318,13 @@ __device__ __forceinline__ int row_wave0(const int* tile_off,
 // reach the row's lanes through DPP broadcasts, while the next tile's loads
 // are in flight. The row's own tile, already in registers, and its Morton
 // neighbour open the walk, so the tests start from a finite k-th distance.
+// A row stops listing tiles once its list holds LC_ROW_FILL entries, and
+// the wave drains: short lists tighten the thresholds before the next tiles
+// are tested (1 to 8 measured alike and faster than 12 to 32 on the
+// clustered, uneven-batch and FCC workloads; 4 best on FCC). At most
+// LC_ROW_FILL - 1 + 16 entries: below LC.
 #ifndef LC_ROW_FILL
-#define LC_ROW_FILL 16
+#define LC_ROW_FILL 4
 #endif
 // Resident 256-thread blocks per CU the register budget is cut for: 3 (3
 // waves per SIMD, 168 VGPRs) for every heap bucket. Measured on the
