@@ -57,7 +57,7 @@ constexpr unsigned long long kFlagPre = 2ull << 62;
 constexpr unsigned long long kValMask = (1ull << 62) - 1;
 constexpr int kFrameNonFinite = 1;
 constexpr long long kInfBits = 0x7ff0000000000000LL;
-constexpr long long kUnusedBits = LLONG_MIN;
+constexpr long long kUnusedBits = static_cast<long long>(0xfff0000000000000ULL);  // -inf
 constexpr unsigned long long kNanBits = 0x7ff8000000000000ull;
 constexpr int kTile = 16;      // points per bounding-box tile
 constexpr int kTileShift = 4;
@@ -462,7 +462,8 @@ __global__ __launch_bounds__(256) void k_cell_box(
 
 // Per-lane top-k in registers, ascending by d2 bits (d2 >= 0 or +inf for
 // finite input, so the IEEE bit pattern orders like the value). Slots
-// [0, KC - k) hold kUnusedBits, below every key, and never leave; slots
+// [0, KC - k) hold kUnusedBits (-inf: below every key both as a double and
+// as a signed integer), and never leave; slots
 // [KC - k, KC) are the k best so far, empty ones (+inf, kEmptyJ), so
 // d[KC - 1] is the k-th best. Only d2 orders the heap: equal distances are
 // resolved at the end (an equal pair inside the final k is ordered by
@@ -483,14 +484,28 @@ __device__ __forceinline__ void top_init(Top<KC>& h, int k) {
   }
 }
 
+// Key minimum and maximum (v_min_f64 / v_max_f64). Heap keys are never
+// NaN, so these order exactly like the compare they replace.
+__device__ __forceinline__ double key_min(double a, double b) {
+  return __builtin_fmin(a, b);
+}
+__device__ __forceinline__ double key_max(double a, double b) {
+  return __builtin_fmax(a, b);
+}
+
+// Compare-exchange of two (d2, index) slots. The keys are ordered as
+// doubles (v_min_f64 / v_max_f64 and one compare for the indices): for the
+// values a heap holds (-inf, d2 >= +0, +inf; never NaN, never -0) that is
+// the order of their bit patterns as signed integers.
 __device__ __forceinline__ void cswap(long long& da, int& ja, long long& db,
                                       int& jb, bool up) {
-  const bool sw = up ? (da > db) : (da < db);
-  const long long td = da;
+  const double a = as_f64(da), b = as_f64(db);
+  const bool sw = up ? (a > b) : (a < b);
+  const double lo = key_min(a, b), hi = key_max(a, b);
+  da = __double_as_longlong(up ? lo : hi);
+  db = __double_as_longlong(up ? hi : lo);
   const int tj = ja;
-  da = sw ? db : da;
   ja = sw ? jb : ja;
-  db = sw ? td : db;
   jb = sw ? tj : jb;
 }
 
@@ -527,16 +542,16 @@ __device__ __forceinline__ void batch_merge(Top<KC>& h, const long long* bd,
     bx[t] = v ? bj[t * kWave] : kEmptyJ;
   }
   bitonic_sort<KC>(b, bx);
+  // empty slots and pads are (+inf, kEmptyJ): a dropped +inf leaves
+  // min_rej unchanged
 #pragma unroll
   for (int t = 0; t < KC; ++t) {
-    const long long x = h.d[t], y = b[KC - 1 - t];
+    const double x = as_f64(h.d[t]), y = as_f64(b[KC - 1 - t]);
     const int xj = h.j[t], yj = bx[KC - 1 - t];
     const bool ty = y < x;
-    h.d[t] = ty ? y : x;
+    h.d[t] = __double_as_longlong(key_min(x, y));
     h.j[t] = ty ? yj : xj;
-    const long long hi = ty ? x : y;
-    const int hij = ty ? xj : yj;
-    if (hij != kEmptyJ) min_rej = fmin(min_rej, as_f64(hi));
+    min_rej = key_min(min_rej, key_max(x, y));
   }
 #pragma unroll
   for (int stride = KC >> 1; stride > 0; stride >>= 1) {
@@ -598,16 +613,16 @@ __device__ __forceinline__ void merge8(Top<KC>& h, const long long* bd,
     bx[t] = v ? bj[t * kWave] : kEmptyJ;
   }
   bitonic_sort<8>(b, bx);
+  // an empty slot or pad is (+inf, kEmptyJ), so the d2 a pair drops feeds
+  // min_rej unconditionally: +inf leaves it unchanged
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
-    const long long x = h.d[KC - 8 + t], y = b[7 - t];
+    const double x = as_f64(h.d[KC - 8 + t]), y = as_f64(b[7 - t]);
     const int xj = h.j[KC - 8 + t], yj = bx[7 - t];
     const bool ty = y < x;
-    h.d[KC - 8 + t] = ty ? y : x;
+    h.d[KC - 8 + t] = __double_as_longlong(key_min(x, y));
     h.j[KC - 8 + t] = ty ? yj : xj;
-    const long long hi = ty ? x : y;
-    const int hij = ty ? xj : yj;
-    if (hij != kEmptyJ) min_rej = fmin(min_rej, as_f64(hi));
+    min_rej = key_min(min_rej, key_max(x, y));
   }
 #pragma unroll
   for (int stride = KC >> 1; stride > 0; stride >>= 1) {
