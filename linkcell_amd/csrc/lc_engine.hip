@@ -771,7 +771,11 @@ enum {
   kStatRowCycMerge, // wave-level cycles merging buffers into the heaps
   kStatRowCycAll,   // wave-level cycles of the whole walk
   kStatRowFlushes,  // wave-level buffer merges
-  kStatCount = 24
+  kStatRowWavesR2,  // waves whose walk reaches shell 2
+  kStatRowRoundsR2, // wave-level tile rounds in shells >= 2
+  kStatRowUsefulR2, // row-level tiles swept in shells >= 2
+  kStatRowCycR2,    // wave-level cycles in shells >= 2 (stats build only)
+  kStatCount = 28
 };
 
 __device__ __forceinline__ void stat_add(unsigned long long* st, int i,
@@ -1481,6 +1485,9 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
            st_cells = 0, st_exact = 0, st_useful = 0;
   unsigned long long cyc_fill = 0, cyc_drain = 0, cyc_merge = 0;
   unsigned st_flush = 0;
+  unsigned st_rounds2 = 0, st_useful2 = 0, st_wave2 = 0;
+  unsigned long long cyc_r2 = 0;
+  int cur_r = 1;
   const unsigned long long cyc0 = kStats ? __builtin_readcyclecounter() : 0;
 
   auto flush = [&]() {
@@ -1617,6 +1624,10 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       cz = nzq + shz;
       cj = njq;
       if (kStats) st_useful += i < lcount ? 1 : 0;
+      if (kStats && cur_r >= 2) {
+        st_rounds2 += 1;
+        st_useful2 += i < lcount ? 1 : 0;
+      }
       i += 1;
       fetch(i);
       if (kStats) st_tiles += 1;
@@ -1629,6 +1640,11 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
 #pragma unroll 1
   for (int r = 1;; ++r) {
     const int ncr = shell_cells(r);
+    cur_r = r;
+    if (kStats && r == 2 && attempt == 0) {
+      st_wave2 = 1;
+      cyc_r2 = __builtin_readcyclecounter();
+    }
     // per-row enumeration state: a chunk of 16 cells, one per lane
     int cc = 0, p0 = 0, total = 0;
     int ca0 = 0, cn2 = 0, ct0 = 0, ccode = 13, cnt_c = 0, cpre = 0;
@@ -1832,7 +1848,11 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       stat_add(A.stats, kStatRowCycMerge, cyc_merge);
       stat_add(A.stats, kStatRowCycAll, __builtin_readcyclecounter() - cyc0);
       stat_add(A.stats, kStatRowFlushes, st_flush);
+      stat_add(A.stats, kStatRowWavesR2, st_wave2);
+      stat_add(A.stats, kStatRowRoundsR2, st_rounds2);
+      if (st_wave2) stat_add(A.stats, kStatRowCycR2, __builtin_readcyclecounter() - cyc_r2);
     }
+    if (li == 0) stat_add(A.stats, kStatRowUsefulR2, st_useful2);
   }
   if (!valid) return;
   const size_t row = static_cast<size_t>(me) * k;
