@@ -775,7 +775,9 @@ enum {
   kStatRowRoundsR2, // wave-level tile rounds in shells >= 2
   kStatRowUsefulR2, // row-level tiles swept in shells >= 2
   kStatRowCycR2,    // wave-level cycles in shells >= 2 (stats build only)
-  kStatCount = 28
+  kStatRowDead,     // row-level tiles swept in which no source accepted
+  kStatRowAccepts,  // lane-level candidates accepted into the buffers
+  kStatCount = 30
 };
 
 __device__ __forceinline__ void stat_add(unsigned long long* st, int i,
@@ -1363,7 +1365,7 @@ template <int KC, bool kStats>
 __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A) {
   constexpr int W = 4;   // waves per block
   constexpr int BC = 8;  // buffered candidates per lane between merges
-  constexpr int LC = 48; // tile list entries per row
+  constexpr int LC = 20; // tile list entries per row (at most kFill - 1 + 16)
   constexpr int kFill = LC_ROW_FILL;  // list length at which a drain starts
   __shared__ long long s_bd[W][BC][kWave];
   __shared__ int s_bj[W][BC][kWave];
@@ -1486,6 +1488,8 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   unsigned long long cyc_fill = 0, cyc_drain = 0, cyc_merge = 0;
   unsigned st_flush = 0;
   unsigned st_rounds2 = 0, st_useful2 = 0, st_wave2 = 0;
+  unsigned st_dead = 0, st_acc = 0;
+  bool tile_acc = false;
   unsigned long long cyc_r2 = 0;
   int cur_r = 1;
   const unsigned long long cyc0 = kStats ? __builtin_readcyclecounter() : 0;
@@ -1532,6 +1536,10 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
         bd[nbuf * kWave] = __double_as_longlong(d2);
         bj[nbuf * kWave] = qj;
         nbuf += 1;
+        if (kStats) {
+          st_acc += 1;
+          tile_acc = true;
+        }
       }
     }
   };
@@ -1631,7 +1639,12 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       i += 1;
       fetch(i);
       if (kStats) st_tiles += 1;
+      if (kStats) tile_acc = false;
       sweep(std::false_type{});
+      if (kStats) {
+        const bool row_any = ((__ballot(tile_acc) >> rbase) & 0xffffull) != 0;
+        st_dead += (i - 1 < lcount && !row_any) ? 1 : 0;
+      }
     }
     lcount = 0;
     if (kStats) cyc_drain += __builtin_readcyclecounter() - t0;
@@ -1783,6 +1796,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
               lslot[pos] = oa0 + t * kTile;
               const int e = on2 - t * kTile;
               lmeta[pos] = ocode | ((e < kTile ? e : kTile) << 5);
+
             }
             lcount += __popc(rm);
             p0 += 16;
@@ -1853,6 +1867,8 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       if (st_wave2) stat_add(A.stats, kStatRowCycR2, __builtin_readcyclecounter() - cyc_r2);
     }
     if (li == 0) stat_add(A.stats, kStatRowUsefulR2, st_useful2);
+    if (li == 0) stat_add(A.stats, kStatRowDead, st_dead);
+    stat_add(A.stats, kStatRowAccepts, st_acc);
   }
   if (!valid) return;
   const size_t row = static_cast<size_t>(me) * k;
