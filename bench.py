@@ -284,7 +284,8 @@ def main():
                          "beside the plain walk)")
     ap.add_argument("--train-frames", type=int, default=32)
     ap.add_argument("--epochs", type=int, default=3)
-    ap.add_argument("--quantile", type=float, default=0.98)
+    ap.add_argument("--quantile", type=float, default=0.999,
+                    help="margin quantile of the training residuals (a higher one re-walks fewer rows)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU sharding is a "

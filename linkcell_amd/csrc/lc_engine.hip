@@ -134,17 +134,31 @@ __device__ __forceinline__ int uni(int v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
-// 2 bits of sub-cell position per axis, interleaved (z, y, x).
+// Sub-cell position of a point inside its cell, LC_SUB_BITS bits per axis
+// (2^LC_SUB_BITS sub-cells per axis), and the Morton key interleaving them
+// (x lowest, then y, then z, per bit). The key orders a cell's members
+// for the walk layout, so 16 consecutive members (a tile) are spatially
+// compact even in cells of several hundred points.
+#ifndef LC_SUB_BITS
+#define LC_SUB_BITS 2
+#endif
 __device__ __forceinline__ int sub_bits(double s, int n, int bin) {
+  constexpr int kSub = 1 << LC_SUB_BITS;
   const double u = s * static_cast<double>(n) - static_cast<double>(bin);
-  int v = static_cast<int>(u * 4.0);
-  v = v < 0 ? 0 : (v > 3 ? 3 : v);
+  int v = static_cast<int>(u * static_cast<double>(kSub));
+  v = v < 0 ? 0 : (v > kSub - 1 ? kSub - 1 : v);
   return v;
 }
 
-__device__ __forceinline__ int morton6(int x, int y, int z) {
-  return (x & 1) | ((y & 1) << 1) | ((z & 1) << 2) | ((x & 2) << 2) |
-         ((y & 2) << 3) | ((z & 2) << 4);
+__device__ __forceinline__ int morton_key(int x, int y, int z) {
+  int key = 0;
+#pragma unroll
+  for (int b = 0; b < LC_SUB_BITS; ++b) {
+    key |= ((x >> b) & 1) << (3 * b);
+    key |= ((y >> b) & 1) << (3 * b + 1);
+    key |= ((z >> b) & 1) << (3 * b + 2);
+  }
+  return key;
 }
 
 // ---------------------------------------------------------------- fill
@@ -190,8 +204,8 @@ __global__ __launch_bounds__(256) void k_fold_count(
   const int bx = bin_axis(s0, d.nx);
   const int by = bin_axis(s1, d.ny);
   const int bz = bin_axis(s2, d.nz);
-  const int key = morton6(sub_bits(s0, d.nx, bx), sub_bits(s1, d.ny, by),
-                          sub_bits(s2, d.nz, bz));
+  const int key = morton_key(sub_bits(s0, d.nx, bx), sub_bits(s1, d.ny, by),
+                             sub_bits(s2, d.nz, bz));
   const long long gcell =
       d.cell_base + (static_cast<long long>(bz) * d.ny + by) * d.nx + bx;
   const int slot = atomicAdd(&count[gcell], 1);

@@ -109,9 +109,15 @@ __device__ __forceinline__ int find_frame(const long long* base, int F,
   return lo;
 }
 
+// Periodic wrap of v = h + d into [0, n), for a home index 0 <= h < n and a
+// neighbourhood offset |d| <= 2 (two steps each way cover n = 1), without
+// the integer division of v % n.
 __device__ __forceinline__ int wrap(int v, int n) {
-  v %= n;
-  return v < 0 ? v + n : v;
+  v = v < 0 ? v + n : v;
+  v = v < 0 ? v + n : v;
+  v = v >= n ? v - n : v;
+  v = v >= n ? v - n : v;
+  return v;
 }
 
 // Per-tile context of the feature map.
@@ -124,7 +130,7 @@ struct TileCtx {
   float pos[3];  // box centre inside the home cell, fractional
 };
 
-__device__ TileCtx tile_ctx(const TileSrc& S, long long T, long long ntiles) {
+__device__ __forceinline__ TileCtx tile_ctx(const TileSrc& S, long long T, long long ntiles) {
   TileCtx c{};
   c.valid = T < ntiles;
   if (!c.valid) return c;
@@ -160,21 +166,46 @@ __device__ TileCtx tile_ctx(const TileSrc& S, long long T, long long ntiles) {
   return c;
 }
 
+// Occupancy input j < kOcc of a tile: the home cell's 5x5x5 periodic
+// neighbourhood, dz -> dy -> dx.
+__device__ __forceinline__ long long occ_index(const TileCtx& c, int j) {
+  const int dz = j / 25 - 2, dy = (j / 5) % 5 - 2, dx = j % 5 - 2;
+  const int x = wrap(c.hx + dx, c.nx);
+  const int y = wrap(c.hy + dy, c.ny);
+  const int z = wrap(c.hz + dz, c.nz);
+  return c.cb + (static_cast<long long>(z) * c.ny + y) * c.nx + x;
+}
+__device__ __forceinline__ float occ_value(int occ) {
+  return log2f(1.0f + static_cast<float>(occ)) * 0.125f;
+}
+__device__ __forceinline__ float occ_feature(const TileSrc& S, const TileCtx& c, int j) {
+  return occ_value(S.count[occ_index(c, j)]);
+}
+
+// The 8 tile inputs kOcc .. kOcc + 7, written at constant indices.
+__device__ __forceinline__ void tile_extra(const TileCtx& c, float (&e)[8]) {
+  e[0] = log2f(c.ext[0] + 1.0f / 256.0f) * 0.125f;
+  e[1] = log2f(c.ext[1] + 1.0f / 256.0f) * 0.125f;
+  e[2] = log2f(c.ext[2] + 1.0f / 256.0f) * 0.125f;
+  e[3] = c.pos[0];
+  e[4] = c.pos[1];
+  e[5] = c.pos[2];
+  e[6] = static_cast<float>(c.fill) * (1.0f / 16.0f);
+  e[7] = log2f(static_cast<float>(c.k)) * 0.125f;
+}
+
 // Input j of a tile (see lc_sched.hpp for the list).
-__device__ float tile_feature(const TileSrc& S, const TileCtx& c, int j) {
+__device__ __forceinline__ float tile_feature(const TileSrc& S, const TileCtx& c, int j) {
   if (!c.valid) return 0.0f;
-  if (j < kOcc) {
-    const int dz = j / 25 - 2, dy = (j / 5) % 5 - 2, dx = j % 5 - 2;
-    const int x = wrap(c.hx + dx, c.nx);
-    const int y = wrap(c.hy + dy, c.ny);
-    const int z = wrap(c.hz + dz, c.nz);
-    const int occ = S.count[c.cb + (static_cast<long long>(z) * c.ny + y) * c.nx + x];
-    return log2f(1.0f + static_cast<float>(occ)) * 0.125f;
+  if (j < kOcc) return occ_feature(S, c, j);
+  if (j < kOcc + 8) {
+    float e[8];
+    tile_extra(c, e);
+    float v = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v = j == kOcc + t ? e[t] : v;
+    return v;
   }
-  if (j < kOcc + 3) return log2f(c.ext[j - kOcc] + 1.0f / 256.0f) * 0.125f;
-  if (j < kOcc + 6) return c.pos[j - kOcc - 3];
-  if (j == kOcc + 6) return static_cast<float>(c.fill) * (1.0f / 16.0f);
-  if (j == kOcc + 7) return log2f(static_cast<float>(c.k)) * 0.125f;
   return 0.0f;
 }
 
@@ -550,13 +581,45 @@ __global__ __launch_bounds__(256) void k_mlp_predict(Net N, TileSrc S,
   const int lr = lane & 15, lg = lane >> 4;
   {
     // four threads per tile, a quarter of the inputs each
-    constexpr int kQ = kFeat / 4;
     const int rr = threadIdx.x >> 2, part = threadIdx.x & 3;
     const TileCtx c = tile_ctx(S, t0 + rr, ntiles);
     if (part == 0) sCM[rr] = static_cast<float>(c.cell_min);
-#pragma unroll 4
-    for (int j = kQ * part; j < kQ * part + kQ; ++j) {
-      sX[rr * kXS + j] = f2bf(tile_feature(S, c, j));
+    // occupancy inputs by (dz, dy) rows of five: thread `part` takes rows
+    // part, part + 4, ...; the five dx offsets wrap once per thread, and
+    // every count load is unconditional (cell 0 stands in for a row a
+    // thread does not have), so all of them are in flight at once
+    int wx[5];
+#pragma unroll
+    for (int d = 0; d < 5; ++d) wx[d] = wrap(c.hx + d - 2, c.nx);
+    int occ[7][5];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      const int r = part + 4 * i;
+      const bool rv = c.valid && r < 25;
+      const int z = wrap(c.hz + r / 5 - 2, c.nz);
+      const int y = wrap(c.hy + r % 5 - 2, c.ny);
+      const long long base = c.cb + (static_cast<long long>(z) * c.ny + y) * c.nx;
+#pragma unroll
+      for (int d = 0; d < 5; ++d) occ[i][d] = S.count[rv ? base + wx[d] : 0];
+    }
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      const int r = part + 4 * i;
+      if (r < 25) {
+#pragma unroll
+        for (int d = 0; d < 5; ++d) {
+          sX[rr * kXS + 5 * r + d] = f2bf(c.valid ? occ_value(occ[i][d]) : 0.0f);
+        }
+      }
+    }
+    static_assert(kOcc == 125 && kOcc + 8 <= kFeat, "5x5x5 occupancy, then the tile inputs");
+    if (part == 3) {
+      float e[8];
+      tile_extra(c, e);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) sX[rr * kXS + kOcc + t] = f2bf(c.valid ? e[t] : 0.0f);
+#pragma unroll
+      for (int j = kOcc + 8; j < kFeat; ++j) sX[rr * kXS + j] = 0;
     }
   }
   __syncthreads();
