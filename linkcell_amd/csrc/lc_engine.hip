@@ -616,16 +616,8 @@ __device__ __forceinline__ void insert_merge(Top<KC>& h, const long long* bd,
 // and min_rej as insert_merge, at a cost that does not grow with the
 // number of entries.
 template <int KC>
-__device__ __forceinline__ void merge8(Top<KC>& h, const long long* bd,
-                                       const int* bj, int n, double& min_rej) {
-  long long b[8];
-  int bx[8];
-#pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    const bool v = t < n;
-    b[t] = v ? bd[t * kWave] : kInfBits;
-    bx[t] = v ? bj[t * kWave] : kEmptyJ;
-  }
+__device__ __forceinline__ void merge8_regs(Top<KC>& h, long long (&b)[8],
+                                            int (&bx)[8], double& min_rej) {
   bitonic_sort<8>(b, bx);
   // an empty slot or pad is (+inf, kEmptyJ), so the d2 a pair drops feeds
   // min_rej unconditionally: +inf leaves it unchanged
@@ -646,6 +638,20 @@ __device__ __forceinline__ void merge8(Top<KC>& h, const long long* bd,
       if (p > i) cswap(h.d[i], h.j[i], h.d[p], h.j[p], true);
     }
   }
+}
+
+template <int KC>
+__device__ __forceinline__ void merge8(Top<KC>& h, const long long* bd,
+                                       const int* bj, int n, double& min_rej) {
+  long long b[8];
+  int bx[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const bool v = t < n;
+    b[t] = v ? bd[t * kWave] : kInfBits;
+    bx[t] = v ? bj[t * kWave] : kEmptyJ;
+  }
+  merge8_regs<KC>(h, b, bx, min_rej);
 }
 
 // Finish a source with a full heap: the row is exact unless a rejected or
@@ -1493,8 +1499,6 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   double rthr = red16_max(live ? thr : -inf);  // -inf: no live source
   int nbuf = 0;
   int done_reach = 0;
-  long long* bd = &s_bd[wv][0][lane];
-  int* bj = &s_bj[wv][0][lane];
   int* lslot = &s_slot[wv][rw][0];
   int* lmeta = &s_meta[wv][rw][0];
   unsigned st_tests = 0, st_tiles = 0, st_merges = 0, st_rounds = 0,
@@ -1514,9 +1518,18 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     if (nbuf > 0) {
       if (kStats) st_merges += 1;
       if constexpr (KC >= 8) {
-        merge8<KC>(h, bd, bj, nbuf, min_rej);
+        // the buffer is read straight from the shared arrays (LDS loads)
+        long long b[8];
+        int bx[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const bool v = t < nbuf;
+          b[t] = v ? s_bd[wv][t][lane] : kInfBits;
+          bx[t] = v ? s_bj[wv][t][lane] : kEmptyJ;
+        }
+        merge8_regs<KC>(h, b, bx, min_rej);
       } else {
-        insert_merge<KC>(h, bd, bj, nbuf, min_rej);
+        insert_merge<KC>(h, &s_bd[wv][0][lane], &s_bj[wv][0][lane], nbuf, min_rej);
       }
       nbuf = 0;
       thr = live ? fmin(as_f64(h.d[KC - 1]), thr_cap) : -inf;
@@ -1547,8 +1560,10 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     if (__ballot(acc)) {
       const int qj = row_bcast<ii>(cj);
       if (acc) {
-        bd[nbuf * kWave] = __double_as_longlong(d2);
-        bj[nbuf * kWave] = qj;
+        // indexed on the shared arrays themselves, so the stores stay LDS
+        // stores whatever the register allocator does with pointers
+        s_bd[wv][nbuf][lane] = __double_as_longlong(d2);
+        s_bj[wv][nbuf][lane] = qj;
         nbuf += 1;
         if (kStats) {
           st_acc += 1;
