@@ -1368,13 +1368,21 @@ __device__ __forceinline__ int row_wave0(const int* tile_off,
 #define LC_ROW_FILL 4
 #endif
 // Resident 256-thread blocks per CU the register budget is cut for: 3 (3
-// waves per SIMD, 168 VGPRs) for every heap bucket. Measured on the
-// clustered 4M (k = 16), the uneven batch and the FCC 8M (k = 32) workloads
-// against 2 and 4: with 4 the spills of the larger heaps cost more than the
-// extra wave hides, with 2 too little latency is hidden.
+// waves per SIMD, 168 VGPRs) for the heaps of up to 16, 2 (256 VGPRs) for
+// the 32-slot heap. Measured on the clustered 4M (k = 16), the uneven batch
+// and the FCC 8M (k = 32) workloads: at 4 the spills of the larger heaps
+// cost more than the extra wave hides, at 2 the k <= 16 walks hide too
+// little latency; the 32-slot heap spills 215 VGPRs at 3 (FCC 16.3 ms)
+// and 2 at 2 (12.7 ms).
+// The row walk's lambdas capture its state by reference; one left out of
+// line would put the heap and thresholds in scratch memory.
+#define LC_INLINE __attribute__((always_inline))
+#ifndef LC_ROW_MINB32
+#define LC_ROW_MINB32 2
+#endif
 template <int KC>
 struct RowOcc {
-  static constexpr int kMinBlocks = 3;
+  static constexpr int kMinBlocks = KC == 32 ? LC_ROW_MINB32 : 3;
 };
 #ifdef LC_ROW_MINB
 #define LC_ROW_BLOCKS(KC) LC_ROW_MINB
@@ -1462,7 +1470,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   const double* hull = s_hull[wv][rw];
 
   // lattice shift of a stencil cell from its wrap code (na+1)*9+(nb+1)*3+nc+1
-  auto shift_of = [&](int code, double& x, double& y, double& z) {
+  auto shift_of = [&](int code, double& x, double& y, double& z) LC_INLINE {
     const int na = code / 9 - 1, nb = (code / 3) % 3 - 1, nc = code % 3 - 1;
     if (code == 13) {
       x = y = z = 0.0;
@@ -1512,7 +1520,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   int cur_r = 1;
   const unsigned long long cyc0 = kStats ? __builtin_readcyclecounter() : 0;
 
-  auto flush = [&]() {
+  auto flush = [&]() LC_INLINE {
     const unsigned long long t0 = kStats ? __builtin_readcyclecounter() : 0;
     if (kStats) st_flush += 1;
     if (nbuf > 0) {
@@ -1547,7 +1555,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   // every lane is active for the DPP read). SELF: the candidates are the
   // row's own sources, and the source itself is skipped; no other tile of
   // the walk holds it (its own tile is swept only in the opening).
-  auto step = [&](auto I, auto selftag) {
+  auto step = [&](auto I, auto selftag) LC_INLINE {
     constexpr int ii = decltype(I)::value;
     constexpr bool SELF = decltype(selftag)::value;
     const double qx = row_bcast<ii>(cx);
@@ -1572,7 +1580,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       }
     }
   };
-  auto pair = [&](auto B, auto selftag) {
+  auto pair = [&](auto B, auto selftag) LC_INLINE {
     constexpr int b0 = decltype(B)::value;
     step(std::integral_constant<int, b0>{}, selftag);
     step(std::integral_constant<int, b0 + 1>{}, selftag);
@@ -1580,7 +1588,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   // the 16 candidates held in cx..cj (one per lane of each row) against
   // every source of the row; one merge site, checked after every two
   // candidates, so buffers fill to BC - 1 or BC before a merge
-  auto sweep = [&](auto selftag) {
+  auto sweep = [&](auto selftag) LC_INLINE {
     if (kStats) st_rounds += 1;
 #pragma unroll 1
     for (int pr = 0; pr < 8; ++pr) {
@@ -1629,12 +1637,12 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   int lcount = 0;  // entries in this row's list
   // drain the list: tile i's candidates are loaded while tile i - 1 is
   // swept
-  auto drain = [&]() {
+  auto drain = [&]() LC_INLINE {
     const unsigned long long t0 = kStats ? __builtin_readcyclecounter() : 0;
     int i = 0;
     double nxq = nan, nyq = nan, nzq = nan;
     int njq = -1, ncode = 13;
-    auto fetch = [&](int e) {
+    auto fetch = [&](int e) LC_INLINE {
       nxq = nyq = nzq = nan;
       njq = -1;
       ncode = 13;
@@ -1789,7 +1797,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
               if (kStats) st_exact += 1;
               const double tq = live ? thr : -inf;
               bool any = false;
-              auto probe_src = [&](auto I) {
+              auto probe_src = [&](auto I) LC_INLINE {
                 constexpr int ii = decltype(I)::value;
                 const double qx = row_bcast<ii>(px);
                 const double qy = row_bcast<ii>(py);
