@@ -56,6 +56,9 @@ constexpr unsigned long long kFlagAgg = 1ull << 62;
 constexpr unsigned long long kFlagPre = 2ull << 62;
 constexpr unsigned long long kValMask = (1ull << 62) - 1;
 constexpr int kFrameNonFinite = 1;
+// The walks' lambdas capture their state (heaps, thresholds, buffers) by
+// reference; one left out of line would put that state in scratch memory.
+#define LC_INLINE __attribute__((always_inline))
 constexpr long long kInfBits = 0x7ff0000000000000LL;
 constexpr long long kUnusedBits = static_cast<long long>(0xfff0000000000000ULL);  // -inf
 constexpr unsigned long long kNanBits = 0x7ff8000000000000ull;
@@ -892,7 +895,7 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_dense(
   unsigned st_tests = 0, st_tiles = 0, st_steps = 0, st_merges = 0,
            st_appends = 0;
 
-  auto flush = [&]() {
+  auto flush = [&]() LC_INLINE {
     st_merges += 1;
     if (nbuf > 0) batch_merge<KC>(h, bd, bj, nbuf, min_rej);
     nbuf = 0;
@@ -904,7 +907,7 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_dense(
   // current ones are used, so the scalar loads overlap the arithmetic.
   // Z: the cell needs no lattice shift (q + 0 == q for the d2 bits).
   auto walk_tiles = [&](auto ztag, int a0, int n2, int t0, int nt, int start,
-                        bool home, double shx, double shy, double shz) {
+                        bool home, double shx, double shy, double shz) LC_INLINE {
     constexpr bool Z = decltype(ztag)::value;
     int t = start;
     Box bx = A.tbox[t0 + t];
@@ -958,7 +961,7 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_dense(
   };
 
   // Visit one candidate cell (unwrapped offset dx, dy, dz from home).
-  auto visit = [&](int dx, int dy, int dz, bool home) {
+  auto visit = [&](int dx, int dy, int dz, bool home) LC_INLINE {
     const int jx = hx + dx, jy = hy + dy, jz = hz + dz;
     const int cx = jx < 0 ? jx + nx : (jx >= nx ? jx - nx : jx);
     const int cy = jy < 0 ? jy + ny : (jy >= ny ? jy - ny : jy);
@@ -1123,7 +1126,7 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
   int* bj = &s_bj[wv][0][lane];
   unsigned st_tests = 0, st_tiles = 0, st_evals = 0, st_merges = 0;
 
-  auto flush = [&]() {
+  auto flush = [&]() LC_INLINE {
     if (nbuf > 0) {
       st_merges += 1;
       batch_merge<KC>(h, bd, bj, nbuf, min_rej);
@@ -1135,7 +1138,7 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
   // One lane's tiles of one candidate cell from tile `start`; the next
   // group of candidates is loaded before the current group is used.
   auto walk_tiles = [&](auto ztag, int a0, int n2, int t0, int nt, int start,
-                        bool home, double shx, double shy, double shz) {
+                        bool home, double shx, double shy, double shz) LC_INLINE {
     constexpr bool Z = decltype(ztag)::value;
 #pragma unroll 1
     for (int tt = 0; tt < nt; ++tt) {
@@ -1182,7 +1185,7 @@ __global__ __launch_bounds__(WalkShape<KC>::kWaves* kWave) void k_walk_sparse(
     }
   };
 
-  auto visit = [&](int dx, int dy, int dz, bool home) {
+  auto visit = [&](int dx, int dy, int dz, bool home) LC_INLINE {
     const int jx = hx + dx, jy = hy + dy, jz = hz + dz;
     const int cx = jx < 0 ? jx + nx : (jx >= nx ? jx - nx : jx);
     const int cy = jy < 0 ? jy + ny : (jy >= ny ? jy - ny : jy);
@@ -1374,9 +1377,6 @@ __device__ __forceinline__ int row_wave0(const int* tile_off,
 // cost more than the extra wave hides, at 2 the k <= 16 walks hide too
 // little latency; the 32-slot heap spills 215 VGPRs at 3 (FCC 16.3 ms)
 // and 2 at 2 (12.7 ms).
-// The row walk's lambdas capture its state by reference; one left out of
-// line would put the heap and thresholds in scratch memory.
-#define LC_INLINE __attribute__((always_inline))
 #ifndef LC_ROW_MINB32
 #define LC_ROW_MINB32 2
 #endif
