@@ -256,7 +256,8 @@ class Workspace:
                   "row_merges", "row_rounds", "row_cells", "row_refines",
                   "row_useful", "row_cyc_fill", "row_cyc_drain", "row_cyc_merge",
                   "row_cyc_all", "row_flushes", "row_waves_r2", "row_rounds_r2",
-                  "row_useful_r2", "row_cyc_r2", "row_dead", "row_accepts")
+                  "row_useful_r2", "row_cyc_r2", "row_dead", "row_accepts",
+                  "row_dead_halves")
 
     def last_stats(self) -> dict:
         n = len(self.STAT_NAMES)

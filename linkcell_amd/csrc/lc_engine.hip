@@ -800,7 +800,8 @@ enum {
   kStatRowCycR2,    // wave-level cycles in shells >= 2 (stats build only)
   kStatRowDead,     // row-level tiles swept in which no source accepted
   kStatRowAccepts,  // lane-level candidates accepted into the buffers
-  kStatCount = 30
+  kStatRowDeadHalf, // row-level 8-candidate halves of swept tiles in which no source accepted
+  kStatCount = 31
 };
 
 __device__ __forceinline__ void stat_add(unsigned long long* st, int i,
@@ -1514,8 +1515,8 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   unsigned long long cyc_fill = 0, cyc_drain = 0, cyc_merge = 0;
   unsigned st_flush = 0;
   unsigned st_rounds2 = 0, st_useful2 = 0, st_wave2 = 0;
-  unsigned st_dead = 0, st_acc = 0;
-  bool tile_acc = false;
+  unsigned st_dead = 0, st_acc = 0, st_deadh = 0;
+  bool tile_acc = false, tile_acc_lo = false, tile_acc_hi = false;
   unsigned long long cyc_r2 = 0;
   int cur_r = 1;
   const unsigned long long cyc0 = kStats ? __builtin_readcyclecounter() : 0;
@@ -1576,6 +1577,8 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
         if (kStats) {
           st_acc += 1;
           tile_acc = true;
+          if (ii < 8) tile_acc_lo = true;
+          else tile_acc_hi = true;
         }
       }
     }
@@ -1676,11 +1679,14 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       i += 1;
       fetch(i);
       if (kStats) st_tiles += 1;
-      if (kStats) tile_acc = false;
+      if (kStats) tile_acc = tile_acc_lo = tile_acc_hi = false;
       sweep(std::false_type{});
       if (kStats) {
         const bool row_any = ((__ballot(tile_acc) >> rbase) & 0xffffull) != 0;
+        const bool row_lo = ((__ballot(tile_acc_lo) >> rbase) & 0xffffull) != 0;
+        const bool row_hi = ((__ballot(tile_acc_hi) >> rbase) & 0xffffull) != 0;
         st_dead += (i - 1 < lcount && !row_any) ? 1 : 0;
+        st_deadh += (i - 1 < lcount) ? (!row_lo ? 1 : 0) + (!row_hi ? 1 : 0) : 0;
       }
     }
     lcount = 0;
@@ -1905,6 +1911,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     }
     if (li == 0) stat_add(A.stats, kStatRowUsefulR2, st_useful2);
     if (li == 0) stat_add(A.stats, kStatRowDead, st_dead);
+    if (li == 0) stat_add(A.stats, kStatRowDeadHalf, st_deadh);
     stat_add(A.stats, kStatRowAccepts, st_acc);
   }
   if (!valid) return;
