@@ -611,6 +611,36 @@ __device__ __forceinline__ void insert_merge(Top<KC>& h, const long long* bd,
   }
 }
 
+// insert_merge over up to 8 candidates already in registers.
+template <int KC>
+__device__ __forceinline__ void insert_merge_regs(Top<KC>& h, const long long (&b)[8],
+                                                  const int (&bx)[8], int n,
+                                                  double& min_rej) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    if (e < n) {
+      const long long x = b[e];
+      const int xj = bx[e];
+      const long long top = h.d[KC - 1];
+      if (x < top) {
+        if (h.j[KC - 1] != kEmptyJ) min_rej = fmin(min_rej, as_f64(top));
+#pragma unroll
+        for (int t = KC - 1; t > 0; --t) {
+          const bool up = h.d[t - 1] > x;
+          const bool here = !up && h.d[t] > x;
+          h.d[t] = up ? h.d[t - 1] : (here ? x : h.d[t]);
+          h.j[t] = up ? h.j[t - 1] : (here ? xj : h.j[t]);
+        }
+        const bool here0 = h.d[0] > x;
+        h.d[0] = here0 ? x : h.d[0];
+        h.j[0] = here0 ? xj : h.j[0];
+      } else {
+        min_rej = fmin(min_rej, as_f64(x));
+      }
+    }
+  }
+}
+
 // Merge up to 8 buffered candidates into the top-k (KC >= 8) with a fixed
 // network: sort the 8 (bitonic), take the element-wise min of the top
 // (ascending) and the batch reversed and padded with +inf (top[KC - 8 + t]
@@ -1396,8 +1426,9 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   constexpr int BC = 8;  // buffered candidates per lane between merges
   constexpr int LC = 20; // tile list entries per row (at most kFill - 1 + 16)
   constexpr int kFill = LC_ROW_FILL;  // list length at which a drain starts
-  __shared__ long long s_bd[W][BC][kWave];
-  __shared__ int s_bj[W][BC][kWave];
+  // buffered candidates: one 16-byte record (d2 bits, index) per slot and
+  // lane, so an append is one address and one ds_write_b128
+  __shared__ int4 s_buf[W][BC][kWave];
   __shared__ int s_slot[W][4][LC];
   __shared__ int s_meta[W][4][LC];
   __shared__ double s_hull[W][4][6];
@@ -1533,12 +1564,24 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
           const bool v = t < nbuf;
-          b[t] = v ? s_bd[wv][t][lane] : kInfBits;
-          bx[t] = v ? s_bj[wv][t][lane] : kEmptyJ;
+          const int4 e = s_buf[wv][t][lane];
+          b[t] = v ? (static_cast<long long>(e.y) << 32) |
+                         static_cast<long long>(static_cast<unsigned>(e.x))
+                   : kInfBits;
+          bx[t] = v ? e.z : kEmptyJ;
         }
         merge8_regs<KC>(h, b, bx, min_rej);
       } else {
-        insert_merge<KC>(h, &s_bd[wv][0][lane], &s_bj[wv][0][lane], nbuf, min_rej);
+        long long b[8];
+        int bx[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          const int4 e = s_buf[wv][t][lane];
+          b[t] = (static_cast<long long>(e.y) << 32) |
+                 static_cast<long long>(static_cast<unsigned>(e.x));
+          bx[t] = e.z;
+        }
+        insert_merge_regs<KC>(h, b, bx, nbuf, min_rej);
       }
       nbuf = 0;
       thr = live ? fmin(as_f64(h.d[KC - 1]), thr_cap) : -inf;
@@ -1571,8 +1614,8 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       if (acc) {
         // indexed on the shared arrays themselves, so the stores stay LDS
         // stores whatever the register allocator does with pointers
-        s_bd[wv][nbuf][lane] = __double_as_longlong(d2);
-        s_bj[wv][nbuf][lane] = qj;
+        const long long db = __double_as_longlong(d2);
+        s_buf[wv][nbuf][lane] = make_int4(static_cast<int>(db), static_cast<int>(db >> 32), qj, 0);
         nbuf += 1;
         if (kStats) {
           st_acc += 1;
