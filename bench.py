@@ -283,7 +283,7 @@ def main():
                          "--train-frames frames of the same mixture, then time it "
                          "beside the plain walk)")
     ap.add_argument("--train-frames", type=int, default=32)
-    ap.add_argument("--epochs", type=int, default=3)
+    ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--quantile", type=float, default=0.999,
                     help="margin quantile of the training residuals (a higher one re-walks fewer rows)")
     args = ap.parse_args()
