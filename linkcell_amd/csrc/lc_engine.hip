@@ -1634,19 +1634,31 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   // the 16 candidates held in cx..cj (one per lane of each row) against
   // every source of the row; one merge site, checked after every two
   // candidates, so buffers fill to BC - 1 or BC before a merge
+  // The switch is entered at pair pr and falls through the following
+  // pairs until a buffer is nearly full; only then does control leave it
+  // for the (single) merge site and come back at the next pair.
   auto sweep = [&](auto selftag) LC_INLINE {
     if (kStats) st_rounds += 1;
+    int pr = 0;
 #pragma unroll 1
-    for (int pr = 0; pr < 8; ++pr) {
+    while (pr < 8) {
       switch (pr) {
-        case 0: pair(std::integral_constant<int, 0>{}, selftag); break;
-        case 1: pair(std::integral_constant<int, 2>{}, selftag); break;
-        case 2: pair(std::integral_constant<int, 4>{}, selftag); break;
-        case 3: pair(std::integral_constant<int, 6>{}, selftag); break;
-        case 4: pair(std::integral_constant<int, 8>{}, selftag); break;
-        case 5: pair(std::integral_constant<int, 10>{}, selftag); break;
-        case 6: pair(std::integral_constant<int, 12>{}, selftag); break;
-        default: pair(std::integral_constant<int, 14>{}, selftag); break;
+#define LC_PAIR(P)                                              \
+  case P:                                                       \
+    pair(std::integral_constant<int, 2 * (P)>{}, selftag);      \
+    pr = (P) + 1;                                               \
+    if (__ballot(nbuf > BC - 2)) break;                         \
+    [[fallthrough]];
+        LC_PAIR(0)
+        LC_PAIR(1)
+        LC_PAIR(2)
+        LC_PAIR(3)
+        LC_PAIR(4)
+        LC_PAIR(5)
+        LC_PAIR(6)
+        LC_PAIR(7)
+#undef LC_PAIR
+        default: break;
       }
       if (__ballot(nbuf > BC - 2)) flush();
     }
