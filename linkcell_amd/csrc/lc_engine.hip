@@ -1436,6 +1436,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   __shared__ int s_slot[W][4][LC];
   __shared__ int s_meta[W][4][LC];
   __shared__ double s_hull[W][4][6];
+  __shared__ double s_shift[W][27][4];  // lattice shift per wrap code (x, y, z, pad)
   const int wv = uni(static_cast<int>(threadIdx.x) / kWave);
   const int lane = threadIdx.x & (kWave - 1);
   const int li = lane & 15;
@@ -1506,20 +1507,37 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   const double* hull = s_hull[wv][rw];
 
   // lattice shift of a stencil cell from its wrap code (na+1)*9+(nb+1)*3+nc+1
-  auto shift_of = [&](int code, double& x, double& y, double& z) LC_INLINE {
-    const int na = code / 9 - 1, nb = (code / 3) % 3 - 1, nc = code % 3 - 1;
-    if (code == 13) {
-      x = y = z = 0.0;
-    } else if (d.ortho) {
-      x = static_cast<double>(na) * d.widths[0];
-      y = static_cast<double>(nb) * d.widths[1];
-      z = static_cast<double>(nc) * d.widths[2];
-    } else {
-      const double fa = na, fb = nb, fc = nc;
-      x = fa * d.h[0] + fb * d.h[3] + fc * d.h[6];
-      y = fa * d.h[1] + fb * d.h[4] + fc * d.h[7];
-      z = fa * d.h[2] + fb * d.h[5] + fc * d.h[8];
+  // The 27 lattice shifts of the frame, from the wrap code
+  // (na+1)*9+(nb+1)*3+nc+1: computed once per wave into LDS (lane = code)
+  // by the expressions of Cell::lattice_shift (src/cell.rs:227-247), so a
+  // lookup gives the same bits without re-reading the frame's cell.
+  {
+    if (lane < 27) {
+      const int code = lane;
+      const int na = code / 9 - 1, nb = (code / 3) % 3 - 1, nc = code % 3 - 1;
+      double x, y, z;
+      if (code == 13) {
+        x = y = z = 0.0;
+      } else if (d.ortho) {
+        x = static_cast<double>(na) * d.widths[0];
+        y = static_cast<double>(nb) * d.widths[1];
+        z = static_cast<double>(nc) * d.widths[2];
+      } else {
+        const double fa = na, fb = nb, fc = nc;
+        x = fa * d.h[0] + fb * d.h[3] + fc * d.h[6];
+        y = fa * d.h[1] + fb * d.h[4] + fc * d.h[7];
+        z = fa * d.h[2] + fb * d.h[5] + fc * d.h[8];
+      }
+      s_shift[wv][code][0] = x;
+      s_shift[wv][code][1] = y;
+      s_shift[wv][code][2] = z;
     }
+    __builtin_amdgcn_wave_barrier();
+  }
+  auto shift_of = [&](int code, double& x, double& y, double& z) LC_INLINE {
+    x = s_shift[wv][code][0];
+    y = s_shift[wv][code][1];
+    z = s_shift[wv][code][2];
   };
 
   Top<KC> h;
