@@ -86,13 +86,38 @@ struct __align__(64) Box {
 // r = 1 stencil in near-first order: home, 6 faces, 12 edges, 8 corners.
 // The canonical heap makes the result independent of the visit order
 // inside a shell; the order only makes the k-th distance tight early.
-__constant__ signed char kNear27[27][3] = {
+constexpr signed char kNear27[27][3] = {
     {0, 0, 0},   {-1, 0, 0},  {1, 0, 0},   {0, -1, 0},  {0, 1, 0},
     {0, 0, -1},  {0, 0, 1},   {-1, -1, 0}, {-1, 1, 0},  {1, -1, 0},
     {1, 1, 0},   {-1, 0, -1}, {-1, 0, 1},  {1, 0, -1},  {1, 0, 1},
     {0, -1, -1}, {0, -1, 1},  {0, 1, -1},  {0, 1, 1},   {-1, -1, -1},
     {-1, -1, 1}, {-1, 1, -1}, {-1, 1, 1},  {1, -1, -1}, {1, -1, 1},
     {1, 1, -1},  {1, 1, 1}};
+
+// The table packed into three 64-bit words, 10 entries of 6 bits each
+// ((dx+1) | (dy+1) << 2 | (dz+1) << 4), so a lane reads its entry with
+// shifts instead of a memory load.
+constexpr unsigned long long near27_word(int w) {
+  unsigned long long v = 0;
+  for (int j = 0; j < 10 && 10 * w + j < 27; ++j) {
+    const int i = 10 * w + j;
+    const unsigned long long e =
+        static_cast<unsigned long long>((kNear27[i][0] + 1) | ((kNear27[i][1] + 1) << 2) |
+                                        ((kNear27[i][2] + 1) << 4));
+    v |= e << (6 * j);
+  }
+  return v;
+}
+__device__ __forceinline__ void near27(int i, int& dx, int& dy, int& dz) {
+  constexpr unsigned long long w0 = near27_word(0), w1 = near27_word(1),
+                               w2 = near27_word(2);
+  const unsigned long long w = i < 10 ? w0 : (i < 20 ? w1 : w2);
+  const int j = i < 10 ? i : (i < 20 ? i - 10 : i - 20);
+  const int e = static_cast<int>((w >> (6 * j)) & 63ull);
+  dx = (e & 3) - 1;
+  dy = ((e >> 2) & 3) - 1;
+  dz = ((e >> 4) & 3) - 1;
+}
 
 void check(hipError_t e, const char* what) {
   if (e != hipSuccess) {
@@ -845,9 +870,7 @@ __device__ __forceinline__ void stat_add(unsigned long long* st, int i,
 __device__ __forceinline__ bool shell_offset(int r, int idx, int& dx, int& dy,
                                              int& dz) {
   if (r == 1) {
-    dx = kNear27[idx][0];
-    dy = kNear27[idx][1];
-    dz = kNear27[idx][2];
+    near27(idx, dx, dy, dz);
     return true;
   }
   const int side = 2 * r + 1;
@@ -1336,9 +1359,7 @@ __device__ __forceinline__ int shell_cells(int r) {
 __device__ __forceinline__ void surface_cell(int r, int i, int& dx, int& dy,
                                              int& dz) {
   if (r == 1) {
-    dx = kNear27[i][0];
-    dy = kNear27[i][1];
-    dz = kNear27[i][2];
+    near27(i, dx, dy, dz);
     return;
   }
   const int s = 2 * r + 1;
