@@ -717,6 +717,14 @@ __device__ __forceinline__ void merge8(Top<KC>& h, const long long* bd,
 // whichever index it met first). Equal d2 inside the row is ordered by
 // index as KHeap::finish does (src/knearest.rs:146-158). Returns false to
 // defer the source.
+// Output rows are written once and never read by the walk: non-temporal
+// stores keep them from evicting candidate tiles (C3 walk 5.02 -> 4.97 ms).
+#ifndef LC_NT_OUT
+#define LC_NT_OUT 1
+#endif
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+
 template <int KC>
 __device__ __forceinline__ bool top_emit(Top<KC>& h, int k, double min_rej,
                                          int* out_nn, double* out_d2,
@@ -740,13 +748,55 @@ __device__ __forceinline__ bool top_emit(Top<KC>& h, int k, double min_rej,
       }
     }
   }
+  // k filling the bucket with 16-byte aligned rows (any k of 4, 8, 16 or
+  // 32 with aligned outputs): whole rows as 16-byte vector stores
+  int* const nn_row = out_nn + row;
+  double* const d2_row = out_d2 != nullptr ? out_d2 + row : nullptr;
+  const bool vec = (KC % 4 == 0) && k == KC &&
+                   (reinterpret_cast<uintptr_t>(nn_row) & 15) == 0 &&
+                   (reinterpret_cast<uintptr_t>(d2_row) & 15) == 0;
+  if (vec) {
+#pragma unroll
+    for (int q = 0; q < KC / 4; ++q) {
+      int v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int t = 4 * q + u;
+        v[u] = h.j[t] != kEmptyJ ? h.j[t] : -1;
+      }
+#if LC_NT_OUT
+      const v4i vv = {v[0], v[1], v[2], v[3]};
+      __builtin_nontemporal_store(vv, reinterpret_cast<v4i*>(nn_row) + q);
+#else
+      reinterpret_cast<int4*>(nn_row)[q] = make_int4(v[0], v[1], v[2], v[3]);
+#endif
+    }
+    if (d2_row != nullptr) {
+#pragma unroll
+      for (int q = 0; q < KC / 2; ++q) {
+        double v[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int t = 2 * q + u;
+          v[u] = h.j[t] != kEmptyJ ? as_f64(h.d[t]) : nan_value();
+        }
+#if LC_NT_OUT
+        const v2d vv = {v[0], v[1]};
+        __builtin_nontemporal_store(vv, reinterpret_cast<v2d*>(d2_row) + q);
+#else
+        reinterpret_cast<double2*>(d2_row)[q] = make_double2(v[0], v[1]);
+#endif
+      }
+    }
+    return true;
+  }
 #pragma unroll
   for (int t = 0; t < KC; ++t) {
     if (t >= KC - k) {
       const int o = t - (KC - k);
       const bool real = h.j[t] != kEmptyJ;
-      out_nn[row + o] = real ? h.j[t] : -1;
-      if (out_d2) out_d2[row + o] = real ? as_f64(h.d[t]) : nan_value();
+      nn_row[o] = real ? h.j[t] : -1;
+      if (d2_row != nullptr) d2_row[o] = real ? as_f64(h.d[t]) : nan_value();
     }
   }
   return true;
