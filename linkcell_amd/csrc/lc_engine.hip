@@ -1384,6 +1384,32 @@ __device__ __forceinline__ int row_bcast(int v) {
   return __builtin_amdgcn_update_dpp(zero, v, 0x150 + I, 0xf, 0xf, true);
 }
 
+// Lane permutation within a 16-lane row through DPP on the two 32-bit
+// halves (64-bit DPP on gfx950 takes only the row broadcast).
+template <int CTRL>
+__device__ __forceinline__ double row_dpp(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) |
+                                        static_cast<long long>(static_cast<unsigned>(lo)));
+}
+// Maximum / minimum over the 16 lanes of a row: quad_perm [1,0,3,2] and
+// [2,3,0,1] reduce each quad, row_half_mirror joins the quads of each half
+// row, row_mirror joins the halves (max and min ignore the order).
+__device__ __forceinline__ double row_max16(double v) {
+  v = fmax(v, row_dpp<0xB1>(v));
+  v = fmax(v, row_dpp<0x4E>(v));
+  v = fmax(v, row_dpp<0x141>(v));
+  return fmax(v, row_dpp<0x140>(v));
+}
+__device__ __forceinline__ double row_min16(double v) {
+  v = fmin(v, row_dpp<0xB1>(v));
+  v = fmin(v, row_dpp<0x4E>(v));
+  v = fmin(v, row_dpp<0x141>(v));
+  return fmin(v, row_dpp<0x140>(v));
+}
+
 // Lower bound of d2 between any point of the source box [sl, sh] and any
 // point of box b shifted by s, with the walk's rounded operations (see
 // box_lb: fl(fl(q + s) - p) is monotone in q and in p, so the bound never
@@ -1559,12 +1585,12 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   // bounding box of the row's sources, kept in LDS (read only while
   // listing tiles)
   {
-    const double v0 = red16_min(valid ? px : inf);
-    const double v1 = red16_max(valid ? px : -inf);
-    const double v2 = red16_min(valid ? py : inf);
-    const double v3 = red16_max(valid ? py : -inf);
-    const double v4 = red16_min(valid ? pz : inf);
-    const double v5 = red16_max(valid ? pz : -inf);
+    const double v0 = row_min16(valid ? px : inf);
+    const double v1 = row_max16(valid ? px : -inf);
+    const double v2 = row_min16(valid ? py : inf);
+    const double v3 = row_max16(valid ? py : -inf);
+    const double v4 = row_min16(valid ? pz : inf);
+    const double v5 = row_max16(valid ? pz : -inf);
     if (li == 0) {
       double* hb = s_hull[wv][rw];
       hb[0] = v0;
@@ -1629,7 +1655,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   bool live = valid;
   double thr = live ? thr_cap : -inf;  // -inf: finished (never accepts)
   bool cap_fail = false;
-  double rthr = red16_max(live ? thr : -inf);  // -inf: no live source
+  double rthr = row_max16(live ? thr : -inf);  // -inf: no live source
   int nbuf = 0;
   int done_reach = 0;
   int* lslot = &s_slot[wv][rw][0];
@@ -1679,7 +1705,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       nbuf = 0;
       thr = live ? fmin(as_f64(h.d[KC - 1]), thr_cap) : -inf;
     }
-    rthr = red16_max(live ? thr : -inf);
+    rthr = row_max16(live ? thr : -inf);
     if (kStats) cyc_merge += __builtin_readcyclecounter() - t0;
   };
 
@@ -1903,14 +1929,15 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
                 }
               }
               cnt_c = (cn2 + kTile - 1) >> kTileShift;
-              // inclusive prefix of tile counts over the row
+              // inclusive prefix of tile counts over the row: DPP row_shr
+              // by 1, 2, 4 and 8 (lanes shifted in from outside the row
+              // read 0), then lane 15's sum broadcast to the row
               cpre = cnt_c;
-#pragma unroll
-              for (int dd = 1; dd < 16; dd <<= 1) {
-                const int y = __shfl_up(cpre, dd, 16);
-                if (li >= dd) cpre += y;
-              }
-              total = __shfl(cpre, rbase + 15);
+              cpre += __builtin_amdgcn_update_dpp(0, cpre, 0x111, 0xf, 0xf, true);
+              cpre += __builtin_amdgcn_update_dpp(0, cpre, 0x112, 0xf, 0xf, true);
+              cpre += __builtin_amdgcn_update_dpp(0, cpre, 0x114, 0xf, 0xf, true);
+              cpre += __builtin_amdgcn_update_dpp(0, cpre, 0x118, 0xf, 0xf, true);
+              total = row_bcast<15>(cpre);
               p0 = 0;
               cc += 16;
             }
@@ -2025,7 +2052,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     // beyond r_safe the stencil wraps onto itself: exact walk
     if (live && r + 1 > d.r_safe) live = false;
     if (!live) thr = -inf;
-    rthr = red16_max(live ? thr : -inf);
+    rthr = row_max16(live ? thr : -inf);
     if (!__ballot(live)) break;
   }
   const bool row_fail = ((__ballot(cap_fail) >> rbase) & 0xffffull) != 0;
@@ -2041,7 +2068,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     done_reach = 0;
     cap_fail = false;
   }
-  rthr = red16_max(live ? thr : -inf);
+  rthr = row_max16(live ? thr : -inf);
   }
   if (kStats) {
     stat_add(A.stats, kStatRowTileTests, st_tests);
