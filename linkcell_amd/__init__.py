@@ -257,7 +257,7 @@ class Workspace:
                   "row_useful", "row_cyc_fill", "row_cyc_drain", "row_cyc_merge",
                   "row_cyc_all", "row_flushes", "row_waves_r2", "row_rounds_r2",
                   "row_useful_r2", "row_cyc_r2", "row_dead", "row_accepts",
-                  "row_dead_halves")
+                  "row_dead_halves", "row_cyc_open", "row_cyc_init")
 
     def last_stats(self) -> dict:
         n = len(self.STAT_NAMES)

@@ -906,7 +906,9 @@ enum {
   kStatRowDead,     // row-level tiles swept in which no source accepted
   kStatRowAccepts,  // lane-level candidates accepted into the buffers
   kStatRowDeadHalf, // row-level 8-candidate halves of swept tiles in which no source accepted
-  kStatCount = 31
+  kStatRowCycOpen,  // wave-level cycles of the opening sweeps (own and next tile)
+  kStatRowCycInit,  // wave-level cycles before the opening
+  kStatCount = 33
 };
 
 __device__ __forceinline__ void stat_add(unsigned long long* st, int i,
@@ -1667,7 +1669,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   unsigned st_rounds2 = 0, st_useful2 = 0, st_wave2 = 0;
   unsigned st_dead = 0, st_acc = 0, st_deadh = 0;
   bool tile_acc = false, tile_acc_lo = false, tile_acc_hi = false;
-  unsigned long long cyc_r2 = 0;
+  unsigned long long cyc_r2 = 0, cyc_open = 0, cyc_init = 0;
   int cur_r = 1;
   const unsigned long long cyc0 = kStats ? __builtin_readcyclecounter() : 0;
 
@@ -1801,22 +1803,31 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   for (int attempt = 0; attempt < 2; ++attempt) {
   // opening: the row's own tile (its sources are the candidates, no
   // shift), then the next tile of the home cell in Morton order
+  const unsigned long long to0 = kStats ? __builtin_readcyclecounter() : 0;
+  if (kStats && attempt == 0) cyc_init = to0 - cyc0;
+  // the next tile's candidates are loaded before the own tile is swept, so
+  // the load overlaps that sweep
+  double nbx = nan, nby = nan, nbz = nan;
+  int nbj = -1;
+  if (has_tile && tl_nb >= 0 && tl_nb * kTile + li < cnt) {
+    const Cand q = A.cand[o0 + tl_nb * kTile + li];
+    nbx = q.x;
+    nby = q.y;
+    nbz = q.z;
+    nbj = static_cast<int>(q.id);
+  }
   cx = valid ? px : nan;
   cy = valid ? py : nan;
   cz = valid ? pz : nan;
   cj = valid ? me : -1;
   sweep(std::true_type{});
-  cx = cy = cz = nan;
-  cj = -1;
-  if (has_tile && tl_nb >= 0 && tl_nb * kTile + li < cnt) {
-    const Cand q = A.cand[o0 + tl_nb * kTile + li];
-    cx = q.x;
-    cy = q.y;
-    cz = q.z;
-    cj = static_cast<int>(q.id);
-  }
+  cx = nbx;
+  cy = nby;
+  cz = nbz;
+  cj = nbj;
   if (__ballot(tl_nb >= 0 && has_tile)) sweep(std::false_type{});
   if (__ballot(nbuf > 0)) flush();
+  if (kStats) cyc_open += __builtin_readcyclecounter() - to0;
 
   int lcount = 0;  // entries in this row's list
   // drain the list: tile i's candidates are loaded while tile i - 1 is
@@ -2090,6 +2101,8 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       stat_add(A.stats, kStatRowWavesR2, st_wave2);
       stat_add(A.stats, kStatRowRoundsR2, st_rounds2);
       if (st_wave2) stat_add(A.stats, kStatRowCycR2, __builtin_readcyclecounter() - cyc_r2);
+      stat_add(A.stats, kStatRowCycOpen, cyc_open);
+      stat_add(A.stats, kStatRowCycInit, cyc_init);
     }
     if (li == 0) stat_add(A.stats, kStatRowUsefulR2, st_useful2);
     if (li == 0) stat_add(A.stats, kStatRowDead, st_dead);
