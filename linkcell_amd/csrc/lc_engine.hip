@@ -1532,8 +1532,9 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   // buffered candidates: one 16-byte record (d2 bits, index) per slot and
   // lane, so an append is one address and one ds_write_b128
   __shared__ int4 s_buf[W][BC][kWave];
-  __shared__ int s_slot[W][4][LC];
-  __shared__ int s_meta[W][4][LC];
+  // tile list entries: (first slot, wrap code | members << 5), one 8-byte
+  // record so a fetch is one LDS read
+  __shared__ int2 s_list[W][4][LC];
   __shared__ double s_hull[W][4][6];
   __shared__ double s_shift[W][27][4];  // lattice shift per wrap code (x, y, z, pad)
   const int wv = uni(static_cast<int>(threadIdx.x) / kWave);
@@ -1660,8 +1661,6 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   double rthr = row_max16(live ? thr : -inf);  // -inf: no live source
   int nbuf = 0;
   int done_reach = 0;
-  int* lslot = &s_slot[wv][rw][0];
-  int* lmeta = &s_meta[wv][rw][0];
   unsigned st_tests = 0, st_tiles = 0, st_merges = 0, st_rounds = 0,
            st_cells = 0, st_exact = 0, st_useful = 0;
   unsigned long long cyc_fill = 0, cyc_drain = 0, cyc_merge = 0;
@@ -1842,8 +1841,9 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       njq = -1;
       ncode = 13;
       if (e < lcount) {
-        const int sl0 = lslot[e];
-        const int meta = lmeta[e];
+        const int2 ent = s_list[wv][rw][e];
+        const int sl0 = ent.x;
+        const int meta = ent.y;
         ncode = meta & 31;
         if (li < (meta >> 5)) {
           const Cand q = A.cand[sl0 + li];
@@ -2029,10 +2029,8 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
             const unsigned rm = static_cast<unsigned>(bal >> rbase) & 0xffffu;
             if (need) {
               const int pos = lcount + __popc(rm & ((1u << li) - 1u));
-              lslot[pos] = oa0 + t * kTile;
               const int e = on2 - t * kTile;
-              lmeta[pos] = ocode | ((e < kTile ? e : kTile) << 5);
-
+              s_list[wv][rw][pos] = make_int2(oa0 + t * kTile, ocode | ((e < kTile ? e : kTile) << 5));
             }
             lcount += __popc(rm);
             p0 += 16;
