@@ -1508,9 +1508,9 @@ __device__ __forceinline__ int row_wave0(const int* tile_off,
 // neighbour open the walk, so the tests start from a finite k-th distance.
 // A row stops listing tiles once its list holds LC_ROW_FILL entries, and
 // the wave drains: short lists tighten the thresholds before the next tiles
-// are tested (1 to 8 measured alike and faster than 12 to 32 on the
-// clustered, uneven-batch and FCC workloads; 4 best on FCC). At most
-// LC_ROW_FILL - 1 + 16 entries: below LC.
+// are tested (2 to 8 measured within 1% of 4 on the clustered 4M frame,
+// 12 to 32 slower; 4 best on the FCC frame). A list holds at most
+// LC_ROW_FILL - 1 + 16 entries, the row walk's list capacity.
 #ifndef LC_ROW_FILL
 #define LC_ROW_FILL 4
 #endif
@@ -1541,8 +1541,11 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   // pays for the heaps of up to 16 (C3 walk 5.27 ms against 5.40 without)
   // and not for the 32-slot heap (FCC 11.15 ms against 10.99)
   constexpr bool kStepBranch = KC < 32;
-  constexpr int LC = 20; // tile list entries per row (at most kFill - 1 + 16)
   constexpr int kFill = LC_ROW_FILL;  // list length at which a drain starts
+  // tile list entries per row: a fill round starts below kFill entries and
+  // appends at most 16
+  constexpr int LC = kFill - 1 + 16;
+  static_assert(kFill >= 1 && LC <= 64, "tile list capacity");
   // buffered candidates: one 16-byte record (d2 bits, index) per slot and
   // lane, so an append is one address and one ds_write_b128
   __shared__ int4 s_buf[W][BC][kWave];
