@@ -146,6 +146,33 @@ def test_points_far_outside_primary_cell(ws):
     assert np.array_equal(nn, on) and same_bits(d2, od)
 
 
+@pytest.mark.parametrize("k", [4, 8, 16, 32])
+def test_unaligned_output_rows_match_aligned(ws, k):
+    """Output rows that are not 16-byte aligned (buffers offset by one
+    element) take the element-wise emit; the results must equal the aligned
+    run bit for bit, and nothing outside the rows may be written."""
+    import torch
+    from gpu_util import cell12
+    rng = np.random.default_rng(40 + k)
+    xyz = rng.random((6000, 3)) * 24.0
+    cell = [24.0, 24.0, 24.0]
+    n = len(xyz)
+    nn_a, d2_a = run_device(ws, xyz, cell, k, None, 2.0)
+    dev = torch.device("cuda:0")
+    dx = torch.from_numpy(np.ascontiguousarray(xyz)).to(dev)
+    nn_buf = torch.full((n * k + 2,), -7, dtype=torch.int32, device=dev)
+    d2_buf = torch.full((n * k + 2,), -7.0, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    ws.knearest_ptr(dx.data_ptr(), n, 1, cell12(cell), k, nn_buf.data_ptr() + 4,
+                    d2_buf.data_ptr() + 8, 0, 2.0, 0, True)
+    nb, db = nn_buf.cpu().numpy(), d2_buf.cpu().numpy()
+    assert nb[0] == -7 and nb[-1] == -7 and db[0] == -7.0 and db[-1] == -7.0
+    assert np.array_equal(nb[1:-1].reshape(n, k), nn_a)
+    assert same_bits(db[1:-1].reshape(n, k), d2_a)
+    on, od = oracle(xyz, cell, k, None, 2.0)
+    assert np.array_equal(nn_a, on) and same_bits(d2_a, od)
+
+
 def test_host_pointer_c_api_matches_oracle(gpu):
     import ctypes as C
     from linkcell_amd import _lib
