@@ -383,6 +383,10 @@ __global__ __launch_bounds__(256) void k_scatter(
   tmp_key[off[cs.x] + cs.y] = (static_cast<long long>(cs.z) << 32) | i;
 }
 
+#ifndef LC_SORT_BATCH
+#define LC_SORT_BATCH 32
+#endif
+
 // One wave per (cell, 64-member chunk). Each lane ranks its member within
 // the segment twice: by (Morton key, id) for the walk layout, whose
 // consecutive members are spatially close, and by id alone for byidx[],
@@ -408,9 +412,21 @@ __global__ __launch_bounds__(256) void k_cell_sort(
   const long long mine = valid ? tmp_key[a] : LLONG_MAX;
   const int mid = static_cast<int>(mine & 0xffffffffll);
   int rk = 0, ri = 0;
-  // the segment's keys eight at a time (wave-uniform addresses: one wide
-  // scalar load per eight keys), then the remainder
+  // the segment's keys LC_SORT_BATCH at a time (wave-uniform addresses:
+  // wide scalar loads, one wait per batch), then eight at a time, then the
+  // remainder
   int b = o0;
+#pragma unroll 1
+  for (; b + LC_SORT_BATCH <= o1; b += LC_SORT_BATCH) {
+    long long v[LC_SORT_BATCH];
+#pragma unroll
+    for (int u = 0; u < LC_SORT_BATCH; ++u) v[u] = tmp_key[b + u];
+#pragma unroll
+    for (int u = 0; u < LC_SORT_BATCH; ++u) {
+      rk += v[u] < mine ? 1 : 0;
+      ri += static_cast<int>(v[u] & 0xffffffffll) < mid ? 1 : 0;
+    }
+  }
 #pragma unroll 1
   for (; b + 8 <= o1; b += 8) {
     long long v[8];
@@ -445,17 +461,34 @@ __global__ __launch_bounds__(256) void k_cell_sort(
   byidx[o0 + ri] = dst;
 }
 
-// Exact bounding boxes. One wave per 64-member chunk = four 16-point
-// tiles; 16-lane groups reduce min / max with shuffles.
-__device__ __forceinline__ double red16_min(double v) {
-  for (int d = 8; d >= 1; d >>= 1) v = fmin(v, __shfl_xor(v, d, kWave));
-  return v;
+// Lane permutation within a 16-lane row through DPP on the two 32-bit
+// halves (64-bit DPP on gfx950 takes only the row broadcast).
+template <int CTRL>
+__device__ __forceinline__ double row_dpp(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) |
+                                        static_cast<long long>(static_cast<unsigned>(lo)));
 }
-__device__ __forceinline__ double red16_max(double v) {
-  for (int d = 8; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, kWave));
-  return v;
+// Maximum / minimum over the 16 lanes of a row: quad_perm [1,0,3,2] and
+// [2,3,0,1] reduce each quad, row_half_mirror joins the quads of each half
+// row, row_mirror joins the halves (max and min ignore the order).
+__device__ __forceinline__ double row_max16(double v) {
+  v = fmax(v, row_dpp<0xB1>(v));
+  v = fmax(v, row_dpp<0x4E>(v));
+  v = fmax(v, row_dpp<0x141>(v));
+  return fmax(v, row_dpp<0x140>(v));
+}
+__device__ __forceinline__ double row_min16(double v) {
+  v = fmin(v, row_dpp<0xB1>(v));
+  v = fmin(v, row_dpp<0x4E>(v));
+  v = fmin(v, row_dpp<0x141>(v));
+  return fmin(v, row_dpp<0x140>(v));
 }
 
+// Exact bounding boxes. One wave per 64-member chunk = four 16-point
+// tiles; each 16-lane row reduces min / max with DPP.
 __global__ __launch_bounds__(256) void k_tile_box(
     const int* __restrict__ items, const int* __restrict__ item_off,
     long long ncells, const int* __restrict__ off,
@@ -470,16 +503,17 @@ __global__ __launch_bounds__(256) void k_tile_box(
   const int q = w - uni(item_off[g]);
   const int o0 = uni(off[g]);
   const int o1 = uni(off[g + 1]);
+  const int t0 = uni(tile_off[g]);
   const int a = o0 + q * kWave + lane;
   const bool valid = a < o1;
   const double inf = as_f64(kInfBits);
   const Cand cr = cand[valid ? a : o0];
   const double x = cr.x, y = cr.y, z = cr.z;
-  const double lx = red16_min(valid ? x : inf), hx = red16_max(valid ? x : -inf);
-  const double ly = red16_min(valid ? y : inf), hy = red16_max(valid ? y : -inf);
-  const double lz = red16_min(valid ? z : inf), hz = red16_max(valid ? z : -inf);
+  const double lx = row_min16(valid ? x : inf), hx = row_max16(valid ? x : -inf);
+  const double ly = row_min16(valid ? y : inf), hy = row_max16(valid ? y : -inf);
+  const double lz = row_min16(valid ? z : inf), hz = row_max16(valid ? z : -inf);
   if ((lane & (kTile - 1)) == 0 && valid) {
-    const long long t = tile_off[g] + (q * kWave + lane) / kTile;
+    const long long t = t0 + (q * kWave + lane) / kTile;
     Box b;
     b.lx = lx;
     b.hx = hx;
@@ -1398,32 +1432,6 @@ template <int I>
 __device__ __forceinline__ int row_bcast(int v) {
   const int zero = 0;
   return __builtin_amdgcn_update_dpp(zero, v, 0x150 + I, 0xf, 0xf, true);
-}
-
-// Lane permutation within a 16-lane row through DPP on the two 32-bit
-// halves (64-bit DPP on gfx950 takes only the row broadcast).
-template <int CTRL>
-__device__ __forceinline__ double row_dpp(double v) {
-  const long long b = __builtin_bit_cast(long long, v);
-  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xf, 0xf, true);
-  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xf, 0xf, true);
-  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) |
-                                        static_cast<long long>(static_cast<unsigned>(lo)));
-}
-// Maximum / minimum over the 16 lanes of a row: quad_perm [1,0,3,2] and
-// [2,3,0,1] reduce each quad, row_half_mirror joins the quads of each half
-// row, row_mirror joins the halves (max and min ignore the order).
-__device__ __forceinline__ double row_max16(double v) {
-  v = fmax(v, row_dpp<0xB1>(v));
-  v = fmax(v, row_dpp<0x4E>(v));
-  v = fmax(v, row_dpp<0x141>(v));
-  return fmax(v, row_dpp<0x140>(v));
-}
-__device__ __forceinline__ double row_min16(double v) {
-  v = fmin(v, row_dpp<0xB1>(v));
-  v = fmin(v, row_dpp<0x4E>(v));
-  v = fmin(v, row_dpp<0x141>(v));
-  return fmin(v, row_dpp<0x140>(v));
 }
 
 // Lower bound of d2 between any point of the source box [sl, sh] and any
