@@ -59,11 +59,10 @@ unsigned blocks_for(long long n, int threads) {
 __host__ __device__ __forceinline__ unsigned short f2bf(float f) {
   unsigned u;
   std::memcpy(&u, &f, sizeof u);
-  if ((u & 0x7fffffffu) > 0x7f800000u) {
-    return static_cast<unsigned short>((u >> 16) | 0x40u);
-  }
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return static_cast<unsigned short>(u >> 16);
+  // both results formed, then selected: no divergent branch per value
+  const unsigned nan = (u >> 16) | 0x40u;
+  const unsigned rne = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+  return static_cast<unsigned short>((u & 0x7fffffffu) > 0x7f800000u ? nan : rne);
 }
 
 __device__ __forceinline__ f32x4 mma(uint4 a, uint4 b, f32x4 c) {
@@ -175,8 +174,10 @@ __device__ __forceinline__ long long occ_index(const TileCtx& c, int j) {
   const int z = wrap(c.hz + dz, c.nz);
   return c.cb + (static_cast<long long>(z) * c.ny + y) * c.nx + x;
 }
+// 1 + occ >= 1 is a normal float, for which v_log_f32 is log2f itself
+// (log2f only adds the rescaling of denormal inputs).
 __device__ __forceinline__ float occ_value(int occ) {
-  return log2f(1.0f + static_cast<float>(occ)) * 0.125f;
+  return __builtin_amdgcn_logf(1.0f + static_cast<float>(occ)) * 0.125f;
 }
 __device__ __forceinline__ float occ_feature(const TileSrc& S, const TileCtx& c, int j) {
   return occ_value(S.count[occ_index(c, j)]);
