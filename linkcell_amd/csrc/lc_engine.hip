@@ -1755,8 +1755,13 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     bool acc = d2 <= thr;
     int qj;
     if constexpr (kStepBranch) {
-      // the index is broadcast only when some lane of the wave accepts
-      if (SELF) acc = acc && row_bcast<ii>(cj) != me;
+      // the index is broadcast only when some lane of the wave accepts.
+      // SELF reads it before the test, with every lane active: a DPP read
+      // from a lane that is switched off returns 0
+      if (SELF) {
+        const int sj = row_bcast<ii>(cj);
+        acc = acc && sj != me;
+      }
       if (!__ballot(acc)) return;
       qj = row_bcast<ii>(cj);
     } else {
