@@ -230,7 +230,7 @@ def bench_batch(args):
             ws.knearest_batch(tr)
             ws.sched_collect()
             del tk
-        loss, margin = ws.sched_train(epochs=args.epochs, batch=4096, lr=2e-3, seed=1,
+        loss, margin = ws.sched_train(epochs=args.epochs, batch=4096, lr=args.lr, seed=1,
                                       quantile=args.quantile)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -283,7 +283,8 @@ def main():
                          "--train-frames frames of the same mixture, then time it "
                          "beside the plain walk)")
     ap.add_argument("--train-frames", type=int, default=32)
-    ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--lr", type=float, default=4e-3, help="Adam learning rate of the scheduler")
     ap.add_argument("--quantile", type=float, default=0.999,
                     help="margin quantile of the training residuals (a higher one re-walks fewer rows)")
     args = ap.parse_args()
@@ -393,7 +394,7 @@ def main():
                     f"{time.perf_counter() - t0:.2f} s")
         samples = ws.sched_samples()
         t1 = time.perf_counter()
-        loss, margin = ws.sched_train(epochs=args.epochs, batch=4096, lr=2e-3, seed=1,
+        loss, margin = ws.sched_train(epochs=args.epochs, batch=4096, lr=args.lr, seed=1,
                                       quantile=args.quantile)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
