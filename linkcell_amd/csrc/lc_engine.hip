@@ -236,7 +236,31 @@ __global__ __launch_bounds__(256) void k_fold_count(
                              sub_bits(s2, d.nz, bz));
   const long long gcell =
       d.cell_base + (static_cast<long long>(bz) * d.ny + by) * d.nx + bx;
-  const int slot = atomicAdd(&count[gcell], 1);
+  // Points given in spatial order (lattices, molecules) put runs of
+  // consecutive lanes in one cell: when every lane is active and the runs
+  // average two lanes or more, each run takes its slots with one atomic from
+  // its first lane. In random order each lane takes its own.
+  const int lane = threadIdx.x & (kWave - 1);
+  int slot = 0;
+  bool runs = false;
+  if (__ballot(true) == ~0ull) {
+    // the previous lane's cell (DPP wave_shr:1; lane 0 reads -1)
+    const int gc = static_cast<int>(gcell);
+    const int gprev = __builtin_amdgcn_update_dpp(-1, gc, 0x138, 0xf, 0xf, false);
+    const bool head = gprev != gc;
+    const unsigned long long heads = __ballot(head);
+    runs = __popcll(heads) <= kWave / 2;
+    if (runs) {
+      const unsigned long long upto = heads & ((2ull << lane) - 1ull);
+      const int start = 63 - __clzll(upto);
+      const unsigned long long above = heads & ~((2ull << lane) - 1ull);
+      const int end = above != 0 ? __ffsll(static_cast<long long>(above)) - 1 : kWave;
+      int base = 0;
+      if (head) base = atomicAdd(&count[gcell], end - lane);
+      slot = __shfl(base, start, kWave) + (lane - start);
+    }
+  }
+  if (!runs) slot = atomicAdd(&count[gcell], 1);
   cellslot[p] = make_int4(static_cast<int>(gcell), slot, key, 0);
 }
 

@@ -125,6 +125,21 @@ def test_perfect_lattice_ties_follow_reference_order(ws, k):
     assert same_bits(d2, od)
 
 
+@pytest.mark.parametrize("masked", [False, True])
+def test_points_given_in_cell_order(ws, masked):
+    # whole waves of consecutive points in one cell take their count slots
+    # one run at a time; a mask leaves lanes idle and takes the per-lane path
+    rng = np.random.default_rng(21)
+    xyz = rng.random((3000, 3)) * 12.0
+    c = np.floor(xyz / 3.0).astype(np.int64)
+    xyz = np.ascontiguousarray(xyz[np.lexsort((c[:, 0], c[:, 1], c[:, 2]))])
+    mask = (rng.random(3000) > 0.1).astype(np.int32) if masked else None
+    nn, d2 = run_device(ws, xyz, [12.0, 12.0, 12.0], 10, mask, 3.0)
+    on, od = oracle(xyz, [12.0, 12.0, 12.0], 10, mask, 3.0)
+    assert np.array_equal(nn, on)
+    assert same_bits(d2, od)
+
+
 def test_coincident_points_keep_zero_distance(ws):
     rng = np.random.default_rng(9)
     base = rng.random((200, 3)) * 9.0
