@@ -289,8 +289,8 @@ def main():
                     help="margin quantile of the training residuals (a higher one re-walks fewer rows)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU; the multi-GPU sharding is a "
-              "design only (see DESIGN.md), so --gpus must be 1", file=sys.stderr)
+        print("bench.py: the search runs on one GPU; sharding it across GPUs is "
+              "designed but not built, so --gpus must be 1", file=sys.stderr)
         raise SystemExit(2)
 
     if args.config == "c4":
