@@ -10,7 +10,7 @@
  *   - lc_last_error is a thread-local slot, NULL after a success;
  *   - lc_version is process-static and leaves the error slot alone.
  *
- * In this build the search itself runs on the GPU (the host buffers are
+ * In this library the search itself runs on the GPU (the host buffers are
  * staged through device memory); results are bit-identical to the
  * reference CPU walk. A process without a usable HIP device gets a
  * nonzero return and the message "HIP device not available".

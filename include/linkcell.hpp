@@ -3,7 +3,7 @@
 // Same names and semantics as the reference header
 // (include/linkcell.hpp:29-248 of linkcell 0.3.3): linkcell::Error,
 // linkcell::Cell, linkcell::Neighbours and the knearest / knearest_into /
-// knearest_into_many free functions. In this build the C API runs on the
+// knearest_into_many free functions. In this library the C API runs on the
 // GPU, so these functions need a HIP device.
 #ifndef LINKCELL_HPP
 #define LINKCELL_HPP
