@@ -412,14 +412,16 @@ __global__ __launch_bounds__(256) void k_scatter(
 #endif
 
 // One wave per (cell, 64-member chunk). Each lane ranks its member within
-// the segment by (Morton key, id) for the walk layout, whose consecutive
-// members are spatially close.
+// the segment twice: by (Morton key, id) for the walk layout, whose
+// consecutive members are spatially close, and by id alone for byidx[],
+// through which the exact-order walk visits a cell in decreasing index
+// order as the reference linked list does (src/knearest.rs:423-428).
 __global__ __launch_bounds__(256) void k_cell_sort(
     const int* __restrict__ items, const int* __restrict__ item_off,
     long long ncells, const int* __restrict__ off,
     const long long* __restrict__ tmp_key, const FrameDesc* __restrict__ fr,
     const long long* __restrict__ fcb, int F, Cand* __restrict__ cand,
-    int* __restrict__ scell) {
+    int* __restrict__ scell, int* __restrict__ byidx) {
   const int w = uni(static_cast<int>(blockIdx.x * (blockDim.x / kWave) +
                                      threadIdx.x / kWave));
   const int lane = threadIdx.x & (kWave - 1);
@@ -433,7 +435,7 @@ __global__ __launch_bounds__(256) void k_cell_sort(
   const bool valid = a < o1;
   const long long mine = valid ? tmp_key[a] : LLONG_MAX;
   const int mid = static_cast<int>(mine & 0xffffffffll);
-  int rk = 0;
+  int rk = 0, ri = 0;
   // the segment's keys LC_SORT_BATCH at a time (wave-uniform addresses:
   // wide scalar loads, one wait per batch), then eight at a time, then the
   // remainder
@@ -446,6 +448,7 @@ __global__ __launch_bounds__(256) void k_cell_sort(
 #pragma unroll
     for (int u = 0; u < LC_SORT_BATCH; ++u) {
       rk += v[u] < mine ? 1 : 0;
+      ri += static_cast<int>(v[u] & 0xffffffffll) < mid ? 1 : 0;
     }
   }
 #pragma unroll 1
@@ -456,11 +459,13 @@ __global__ __launch_bounds__(256) void k_cell_sort(
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       rk += v[u] < mine ? 1 : 0;
+      ri += static_cast<int>(v[u] & 0xffffffffll) < mid ? 1 : 0;
     }
   }
   for (; b < o1; ++b) {
     const long long v = tmp_key[b];
     rk += v < mine ? 1 : 0;
+    ri += static_cast<int>(v & 0xffffffffll) < mid ? 1 : 0;
   }
   if (!valid) return;
   const int f = find_frame(fcb, F, g);
@@ -477,46 +482,7 @@ __global__ __launch_bounds__(256) void k_cell_sort(
   cr.id = mid;
   cand[dst] = cr;
   scell[dst] = g;
-}
-
-// Decreasing-index order of each cell for the exact-order walk, which
-// visits a cell's members as the reference linked list does
-// (src/knearest.rs:423-428): byidx[o0 + r] is the walk-layout position of
-// the member whose index has rank r in the cell. One wave per (cell,
-// 64-member chunk), ranking by the ids of the sorted records. Only the
-// exact-order walk reads it, so every wave first reads the deferred-source
-// count and leaves when it is zero (a search with no deferred source, the
-// common case, skips the ranking).
-__global__ __launch_bounds__(256) void k_cell_rank_id(
-    const int* __restrict__ items, const int* __restrict__ item_off,
-    long long ncells, const int* __restrict__ off,
-    const Cand* __restrict__ cand, const int* __restrict__ defer_count,
-    int* __restrict__ byidx) {
-  if (uni(*defer_count) == 0) return;
-  const int w = uni(static_cast<int>(blockIdx.x * (blockDim.x / kWave) +
-                                     threadIdx.x / kWave));
-  const int lane = threadIdx.x & (kWave - 1);
-  const int total = item_off[ncells];
-  if (w >= total) return;
-  const int g = uni(items[w]);
-  const int q = w - uni(item_off[g]);
-  const int o0 = uni(off[g]);
-  const int o1 = uni(off[g + 1]);
-  const int a = o0 + q * kWave + lane;
-  const bool valid = a < o1;
-  const int mid = valid ? static_cast<int>(cand[a].id) : INT_MAX;
-  int ri = 0;
-  int b = o0;
-#pragma unroll 1
-  for (; b + 8 <= o1; b += 8) {
-    int v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = static_cast<int>(cand[b + u].id);
-#pragma unroll
-    for (int u = 0; u < 8; ++u) ri += v[u] < mid ? 1 : 0;
-  }
-  for (; b < o1; ++b) ri += static_cast<int>(cand[b].id) < mid ? 1 : 0;
-  if (valid) byidx[o0 + ri] = a;
+  byidx[o0 + ri] = dst;
 }
 
 // Lane permutation within a 16-lane row through DPP on the two 32-bit
@@ -2960,7 +2926,7 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   k_cell_sort<<<item_blocks, 256, 0, st>>>(
       as<int>(items_.p), as<int>(item_off_.p), ctot, as<int>(off_.p),
       as<long long>(tmp_key_.p), dfr, dfcb, F, as<Cand>(cand_.p),
-      as<int>(scell_.p));
+      as<int>(scell_.p), as<int>(byidx_.p));
   k_tile_box<<<item_blocks, 256, 0, st>>>(
       as<int>(items_.p), as<int>(item_off_.p), ctot, as<int>(off_.p),
       as<int>(tile_off_.p), as<Cand>(cand_.p), as<Box>(tbox_.p));
@@ -3084,9 +3050,6 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
     check(hipGetLastError(), "route launch");
   }
   if (timing) check(hipEventRecord(ev_[5], st), "event record");
-  k_cell_rank_id<<<item_blocks, 256, 0, st>>>(
-      as<int>(items_.p), as<int>(item_off_.p), ctot, as<int>(off_.p),
-      as<Cand>(cand_.p), counters + 1, as<int>(byidx_.p));
   k_walk_exact_wave<<<2048, 64, 0, st>>>(
       dfr, dfcb, F, as<Cand>(cand_.p), as<int>(scell_.p), as<int>(off_.p),
       as<int>(byidx_.p), as<int>(defer_.p), counters + 1);
