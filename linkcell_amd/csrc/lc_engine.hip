@@ -168,7 +168,7 @@ __device__ __forceinline__ int uni(int v) {
 // for the walk layout, so 16 consecutive members (a tile) are spatially
 // compact even in cells of several hundred points.
 #ifndef LC_SUB_BITS
-#define LC_SUB_BITS 2
+#define LC_SUB_BITS 3
 #endif
 __device__ __forceinline__ int sub_bits(double s, int n, int bin) {
   constexpr int kSub = 1 << LC_SUB_BITS;
