@@ -2,15 +2,16 @@
 //
 // Preparation (all single-pass streaming kernels):
 //   fold_count  fold every point into its primary cell, bin it, count
-//               occupants with one atomic per point (the slot it returns is
-//               the point's arrival rank in the cell); a 6-bit Morton key of
-//               the point's 4x4x4 sub-cell rides along
+//               occupants with one atomic per point, or per run of lanes in
+//               one cell (the slot is the point's arrival rank in the
+//               cell); the 9-bit Hilbert index of the point's 8x8x8
+//               sub-cell rides along
 //   scan x2     single-pass decoupled look-back exclusive scans over cell
 //               counts; one packed 64-bit status word per 2048-cell tile
 //               carries the occupant prefix and a second prefix (16-point
 //               tiles, then 64-source chunks)
 //   scatter     arrival-order scatter of (key, id) into cell segments
-//   cell_sort   one wave per 64-member chunk: rank by (Morton key, id) for
+//   cell_sort   one wave per 64-member chunk: rank by (sub-cell key, id) for
 //               the walk layout and by id for the exact-order replay,
 //               re-fold, write SoA FP64 coordinates
 //   boxes       exact bounding boxes of every 16-point tile and every cell
@@ -163,10 +164,11 @@ __device__ __forceinline__ int uni(int v) {
 }
 
 // Sub-cell position of a point inside its cell, LC_SUB_BITS bits per axis
-// (2^LC_SUB_BITS sub-cells per axis), and the Morton key interleaving them
-// (x lowest, then y, then z, per bit). The key orders a cell's members
-// for the walk layout, so 16 consecutive members (a tile) are spatially
-// compact even in cells of several hundred points.
+// (2^LC_SUB_BITS sub-cells per axis), and its key: the Hilbert index below
+// by default, or the Morton key interleaving the bits (x lowest, then y,
+// then z, per bit) with LC_HILBERT=0. The key orders a cell's members for
+// the walk layout, so 16 consecutive members (a tile) are spatially compact
+// even in cells of several hundred points.
 #ifndef LC_SUB_BITS
 #define LC_SUB_BITS 3
 #endif
@@ -456,7 +458,7 @@ __global__ __launch_bounds__(256) void k_scatter(
 #endif
 
 // One wave per (cell, 64-member chunk). Each lane ranks its member within
-// the segment twice: by (Morton key, id) for the walk layout, whose
+// the segment twice: by (sub-cell key, id) for the walk layout, whose
 // consecutive members are spatially close, and by id alone for byidx[],
 // through which the exact-order walk visits a cell in decreasing index
 // order as the reference linked list does (src/knearest.rs:423-428).
@@ -1058,7 +1060,7 @@ struct WalkShape {
 
 // ---------------------------------------------------------- dense walk
 
-// One wave = up to 64 sources of one home cell (sources in Morton order,
+// One wave = up to 64 sources of one home cell (sources in sub-cell order,
 // so the wave covers a compact part of the cell). Per candidate cell and
 // tile, each lane tests the tile's exact lower bound against its k-th
 // best; the wave skips the tile unless some lane needs it. Candidates are
@@ -1571,8 +1573,9 @@ __device__ __forceinline__ int row_wave0(const int* tile_off,
   return (tile_off[fcb[f]] + 3) / 4 + f;
 }
 
-// One 16-lane row per source tile (16 Morton-consecutive sources of one
-// home cell), four rows per wave, every wave inside one frame. Per shell,
+// One 16-lane row per source tile (16 consecutive sources of one home
+// cell, in sub-cell order), four rows per wave, every wave inside one
+// frame. Per shell,
 // a row lists the candidate tiles some of its sources still need: 16 cells
 // (one per lane) get their headers at once, then 16 of their tiles (one per
 // lane) get a bounding-box test against the row's sources and, when that
@@ -1580,8 +1583,8 @@ __device__ __forceinline__ int row_wave0(const int* tile_off,
 // list, in LDS, is then drained tile by tile: each lane loads one
 // candidate of the tile, adds the lattice shift, and the 16 candidates
 // reach the row's lanes through DPP broadcasts, while the next tile's loads
-// are in flight. The row's own tile, already in registers, and its Morton
-// neighbour open the walk, so the tests start from a finite k-th distance.
+// are in flight. The row's own tile, already in registers, and the next tile
+// of its cell open the walk, so the tests start from a finite k-th distance.
 // A row stops listing tiles once its list holds LC_ROW_FILL entries, and
 // the wave drains: short lists tighten the thresholds before the next tiles
 // are tested (2 to 8 measured within 1% of 4 on the clustered 4M frame,
@@ -1899,7 +1902,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
 #pragma unroll 1
   for (int attempt = 0; attempt < 2; ++attempt) {
   // opening: the row's own tile (its sources are the candidates, no
-  // shift), then the next tile of the home cell in Morton order
+  // shift), then the next tile of the home cell in sub-cell order
   const unsigned long long to0 = kStats ? __builtin_readcyclecounter() : 0;
   if (kStats && attempt == 0) cyc_init = to0 - cyc0;
   // the next tile's candidates are loaded before the own tile is swept, so
