@@ -103,6 +103,17 @@ def walk_bytes(cid, plan, reach, k):
     return n_act * (24 + 4 + 12 * k) + BYTES_PER_CANDIDATE * total, n_act
 
 
+def cpu_model():
+    """The host CPU's model string (/proc/cpuinfo), as lscpu prints it."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(xyz, cell, k, hint, mask, target_s=12.0):
     """Oracle (C restatement of the reference CPU walk) on this host's
     cores: full fold/bin/list preparation, the walk on a random sample of
@@ -124,7 +135,7 @@ def cpu_baseline(xyz, cell, k, hint, mask, target_s=12.0):
         m = m2
     t_full = prep + walk * (n / m)
     return {"value": n / t_full, "unit": "source-queries/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(),
             "sample": (f"C restatement of the reference CPU walk, {threads} threads: fold/bin/"
                        f"lists over all {n} points ({prep:.2f} s) + walk of {m} random "
                        f"sources ({walk:.2f} s), extrapolated to {n} sources")}
@@ -282,7 +293,8 @@ def main():
                     help="headline schedule (learned: train the scheduler on "
                          "--train-frames frames of the same mixture, then time it "
                          "beside the plain walk)")
-    ap.add_argument("--train-frames", type=int, default=32)
+    ap.add_argument("--train-frames", type=int, default=None,
+                    help="training frames of the learned leg (default 32 for c3, 8 for c5ii)")
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--lr", type=float, default=4e-3, help="Adam learning rate of the scheduler")
     ap.add_argument("--quantile", type=float, default=0.999,
@@ -325,25 +337,33 @@ def main():
                         mptr, hint, rr.data_ptr() if reach else 0, wait)
 
     def leg(tag):
-        """warmup, K timed steps, per-kernel hipEvent timings, work counters"""
+        """warmup, then K timed steps with hipEvents on the workspace stream
+        around every stage of every step (medians over the K steps), then
+        one step with the work counters on"""
         for _ in range(args.warmup):
             step()
         ws.wait()
         torch.cuda.synchronize()
+        ws.set_timing(True)
         t0 = time.perf_counter()
         for _ in range(args.steps):
             step()
         ws.wait()
         torch.cuda.synchronize()
         ms_step = (time.perf_counter() - t0) * 1e3 / args.steps
-        log(f"bench: {tag}: timed {args.steps} steps, {ms_step:.3f} ms/step")
-        ws.set_timing(True)
-        kt = []
-        for _ in range(max(3, min(args.steps, 10))):
-            step(wait=True)
-            kt.append(ws.last_timing())
+        hist = ws.timing_history(args.steps)
         ws.set_timing(False)
-        kern = {key: float(np.mean([t[key] for t in kt])) for key in kt[0]}
+        log(f"bench: {tag}: timed {args.steps} steps, {ms_step:.3f} ms/step")
+        kern = {key: float(np.median([t[key] for t in hist])) for key in hist[0]}
+        kern["walk_exact"] = float(np.median([t["walk"] + t["exact"] for t in hist]))
+        kern["prep"] = float(np.median([t["fold"] + t["scan"] + t["scatter"] + t["sort"]
+                                        for t in hist]))
+        kern["steps_timed"] = len(hist)
+        ws.set_timing(True)
+        step(wait=True)
+        last = ws.last_timing()
+        ws.set_timing(False)
+        kern["deferred"], kern["retried"] = last["deferred"], last["retried"]
         ws.set_stats(True)
         step(wait=True)
         work = ws.last_stats()
@@ -362,27 +382,39 @@ def main():
     kc = 4 if k <= 4 else 8 if k <= 8 else 16 if k <= 16 else 32 if k <= 32 else 0
 
     def roof(kern):
-        t_walk = (kern["walk"] + kern["exact"]) * 1e-3
+        t_walk = kern["walk_exact"] * 1e-3
         achieved = bwalk / t_walk / 1e9
         return t_walk, achieved
+
+    # preparation stage (fold, count, scan, scatter, cell sort, boxes):
+    # streaming, B_prep = N (124 + 4 with a mask) + 12 nC bytes
+    ncell = plan["nx"] * plan["ny"] * plan["nz"]
+    bprep = n * (124 + (4 if mask is not None else 0)) + 12 * ncell
+
+    def prep_of(kern):
+        t = kern["prep"] * 1e-3
+        return {"bytes": bprep, "ms": kern["prep"], "achieved_GBs": bprep / t / 1e9,
+                "frac": bprep / t / 1e9 / HBM_PEAK_GBS}
 
     t_walk_plain, ach_plain = roof(kern_plain)
     plain = {"value": n_act / (ms_plain * 1e-3), "ms_per_step": ms_plain,
              "walk_ms": t_walk_plain * 1e3, "walk_qps": n_act / t_walk_plain,
              "roofline_frac": ach_plain / HBM_PEAK_GBS, "kernels_ms": kern_plain,
-             "walk_work": work_plain}
+             "prep": prep_of(kern_plain), "walk_work": work_plain}
 
     learned = None
-    learned_ok = (args.schedule == "learned" and args.config == "c3" and kc != 0
+    learned_ok = (args.schedule == "learned" and args.config in TRAIN_FRAMES and kc != 0
                   and split == -1 and args.mode == 0)
+    train_frames = (args.train_frames if args.train_frames is not None
+                    else TRAIN_FRAMES.get(args.config, 0))
     if learned_ok:
         ref_nn, ref_d2 = nn.clone(), d2.clone()
         tn = torch.empty((n, k), dtype=torch.int32, device=dev)
         td = torch.empty((n, k), dtype=torch.float64, device=dev)
         t0 = time.perf_counter()
         ws.sched_clear()
-        for f in range(1, args.train_frames + 1):
-            tx = wl_device_frame(f, dev)
+        for f in range(1, train_frames + 1):
+            tx = wl_device_frame(args.config, f, dev)
             # drawn on torch's stream; the workspace searches on its own
             torch.cuda.synchronize()
             ws.knearest_ptr(tx.data_ptr(), n, 1, c12, k, tn.data_ptr(), td.data_ptr(),
@@ -399,7 +431,7 @@ def main():
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         del tn, td
-        log(f"bench: learned scheduler: {args.train_frames} frames, {samples} tiles "
+        log(f"bench: learned scheduler: {train_frames} frames, {samples} tiles "
             f"collected in {t1 - t0:.2f} s, trained in {t2 - t1:.2f} s "
             f"(loss {loss:.4f}, margin {margin:.3f})")
         ws.set_schedule(1)
@@ -415,7 +447,7 @@ def main():
         ws.set_timing(False)
         ntiles = ws.tile_count()
         learned = {"ms_per_step": ms_l, "kernels_ms": kern_l, "walk_work": work_l,
-                   "train_frames": args.train_frames, "samples": samples,
+                   "train_frames": train_frames, "samples": samples,
                    "epochs": args.epochs, "loss": loss, "margin_log2": margin,
                    "quantile": args.quantile, "collect_s": t1 - t0, "train_s": t2 - t1,
                    "retried_rows": retried, "rows": ntiles,
@@ -463,7 +495,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": bwalk,
-                     "kernel": walk_kernel},
+                     "kernel": walk_kernel,
+                     "timing": f"hipEvent median over the {kern['steps_timed']} timed steps"},
+        "prep": prep_of(kern),
         "plain": plain,
     }
     if learned is not None:
@@ -479,8 +513,14 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def wl_device_frame(f, dev):
+# training frames of the learned leg per workload (drawn on the device)
+TRAIN_FRAMES = {"c3": 32, "c5ii": 8}
+
+
+def wl_device_frame(config, f, dev):
     from linkcell_amd import workloads as wl
+    if config == "c5ii":
+        return wl.fcc_c5ii_device(5500 + f, dev)
     return wl.clustered_c3_device(f, dev)
 
 

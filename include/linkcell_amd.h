@@ -132,6 +132,25 @@ int lc_gpu_sched_tile_count(lc_gpu_workspace *ws, size_t *n_tiles);
  * be valid. Waits for completion. */
 int lc_gpu_sched_tile_labels(lc_gpu_workspace *ws, float *out, size_t n);
 
+/* Stage timings (milliseconds) of up to n_runs of the most recent timed
+ * searches of this workspace, oldest first, 10 per run: fold, scan,
+ * scatter, sort, walk, exact, total, schedule set-up, first walk pass,
+ * retry; *got receives the number of runs written (at most 64 are kept).
+ * Waits for the latest. */
+int lc_gpu_timing_history(lc_gpu_workspace *ws, double *ms, int n_runs, int *got);
+
+/* Diagnostics of the learned scheduler on the last search (host arrays):
+ * x_collect (n_tiles x 160 BF16 bit patterns) are the inputs the training
+ * set takes for every tile; x_predict (same shape) and caps (n_tiles
+ * floats) are the inputs and caps of inference, written only when the
+ * predictor is trained (either may be NULL to skip them); tile_cell (home
+ * cell per tile), tile_box (8 doubles per tile: lx, hx, ly, hy, lz, hz,
+ * pad, pad) and count (occupancy per cell, n_cells ints) are the layout
+ * they come from. n_tiles and n_cells must cover the search. Waits. */
+int lc_gpu_sched_dump(lc_gpu_workspace *ws, size_t n_tiles, size_t n_cells,
+                      unsigned short *x_collect, unsigned short *x_predict,
+                      float *caps, int *tile_cell, double *tile_box, int *count);
+
 /* Rows of the last timed search that were walked again without their
  * predicted cap (needs timing on, like lc_gpu_last_timing). */
 int lc_gpu_last_retried(lc_gpu_workspace *ws, int *retried);

@@ -17,6 +17,7 @@ HIP kernels; results are bit-identical to the reference CPU walk.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 
 import numpy as np
 
@@ -140,6 +141,18 @@ class Workspace:
         out["retried"] = int(ms[11])
         return out
 
+    TIMING_KEYS = ("fold", "scan", "scatter", "sort", "walk", "exact", "total",
+                   "setup", "pass1", "retry")
+
+    def timing_history(self, n: int) -> list:
+        """Stage milliseconds of up to n of the most recent timed searches,
+        oldest first (the workspace keeps 64)."""
+        ms = (C.c_double * (10 * max(1, n)))()
+        got = C.c_int(0)
+        self._check(self._lib.lc_gpu_timing_history(self._ws, ms, int(n), C.byref(got)))
+        return [{k: float(ms[10 * i + j]) for j, k in enumerate(self.TIMING_KEYS)}
+                for i in range(got.value)]
+
     def set_dense_min(self, v: int):
         self._check(self._lib.lc_gpu_set_dense_min(self._ws, int(v)))
 
@@ -171,6 +184,25 @@ class Workspace:
         """Largest final k-th-neighbour distance per source tile of the last
         search into a device float32 array."""
         self._check(self._lib.lc_gpu_sched_tile_labels(self._ws, C.c_void_p(out_ptr), int(n)))
+
+    def sched_dump(self, n_cells: int, predict: bool = True) -> dict:
+        """Scheduler diagnostics of the last search: the BF16 inputs of
+        every tile as training collects them and (when trained, with
+        predict) as inference reads them, inference's caps, and the tile
+        layout (home cell, box, cell occupancy)."""
+        nt = self.tile_count()
+        xc = np.zeros((nt, 160), dtype=np.uint16)
+        xp = np.zeros((nt, 160), dtype=np.uint16) if predict else None
+        caps = np.zeros(nt, dtype=np.float32) if predict else None
+        cells = np.zeros(nt, dtype=np.int32)
+        box = np.zeros((nt, 8), dtype=np.float64)
+        count = np.zeros(n_cells, dtype=np.int32)
+        self._check(self._lib.lc_gpu_sched_dump(
+            self._ws, nt, n_cells, xc.ctypes.data, xp.ctypes.data if predict else None,
+            caps.ctypes.data if predict else None, cells.ctypes.data, box.ctypes.data,
+            count.ctypes.data))
+        return dict(x_collect=xc, x_predict=xp, caps=caps, tile_cell=cells, tile_box=box,
+                    count=count)
 
     def last_retried(self) -> int:
         """Rows of the last timed search walked again without their cap."""
@@ -334,14 +366,18 @@ class Workspace:
         return int(best.value), float(ms.value)
 
 
-_default_ws = None
+_tls = threading.local()
 
 
 def _workspace() -> Workspace:
-    global _default_ws
-    if _default_ws is None:
-        _default_ws = Workspace()
-    return _default_ws
+    """This thread's workspace: each thread searches on its own stream and
+    scratch (the reference binding creates one per call,
+    python/src/gpu.rs:239; one per thread keeps the grow-only buffers)."""
+    ws = getattr(_tls, "ws", None)
+    if ws is None:
+        ws = Workspace()
+        _tls.ws = ws
+    return ws
 
 
 def _host_mask(mask, n):
@@ -398,6 +434,10 @@ def knearest(xyz, cell, k, mask=None, cell_hint=None):
 
 
 def _knearest_device(xyz, cell, k, mask, hint):
+    """Device search on the thread's workspace stream, ordered after
+    torch's current stream by an event and before it by another (the
+    reference binding orders on lc_gpu_queue, python/src/gpu.rs:252); the
+    host does not wait."""
     import torch
 
     if not _is_torch(xyz):
@@ -408,29 +448,49 @@ def _knearest_device(xyz, cell, k, mask, hint):
     xyz = xyz.contiguous()
     dev = xyz.device
     shape = (n, k) if frames == 1 else (frames, n, k)
-    nn = torch.empty(shape, dtype=torch.int32, device=dev)
-    d2 = torch.empty(shape, dtype=torch.float64, device=dev)
     m_dev = None
     if mask is not None:
         if _is_torch(mask) and mask.is_cuda:
-            m_dev = mask.to(torch.float64) if mask.dtype == torch.float64 else mask
-            m_dev = (m_dev != 0).to(torch.int32).contiguous() if m_dev.dtype != torch.int32 else m_dev.contiguous()
+            # python/src/lib.rs:188-211: int32 as given, float64 nonzero keeps
+            if mask.dtype == torch.int32:
+                m_dev = mask.contiguous()
+            elif mask.dtype == torch.float64:
+                m_dev = (mask != 0).to(torch.int32).contiguous()
+            else:
+                raise ValueError("mask must be int32 or float64")
         else:
             m_dev = torch.from_numpy(_host_mask(mask, n)).to(dev)
         if m_dev.numel() != n:
             raise ValueError("mask length must be n")
-    ws = _workspace()
-    torch.cuda.current_stream(dev).synchronize()
+    dcell = None
     if _is_torch(cell) and cell.is_cuda:
-        c = cell.to(torch.float64).contiguous()
-        if tuple(c.shape) not in ((3,), (3, 3), (4, 3)):
+        if cell.dtype != torch.float64 or tuple(cell.shape) not in ((3,), (3, 3), (4, 3)):
             raise ValueError("cell must be float64 shape (3,) ortho, (3, 3) lattice rows, "
                              "or (4, 3) rows plus origin")
-        ws.knearest_dcell_ptr(xyz.data_ptr(), n, frames, c.data_ptr(), c.numel(), k,
-                              nn.data_ptr(), d2.data_ptr(),
-                              m_dev.data_ptr() if m_dev is not None else 0, hint)
+        dcell = cell.contiguous()
     else:
-        ws.knearest_ptr(xyz.data_ptr(), n, frames, _cell12(cell), k, nn.data_ptr(),
-                        d2.data_ptr(), m_dev.data_ptr() if m_dev is not None else 0,
-                        hint)
+        c12 = _cell12(cell)
+    nn = torch.empty(shape, dtype=torch.int32, device=dev)
+    d2 = torch.empty(shape, dtype=torch.float64, device=dev)
+    ws = _workspace()
+    cur = torch.cuda.current_stream(dev)
+    queue = torch.cuda.ExternalStream(ws.queue, device=dev)
+    ready = torch.cuda.Event()
+    ready.record(cur)
+    queue.wait_event(ready)
+    mptr = m_dev.data_ptr() if m_dev is not None else 0
+    if dcell is not None:
+        ws.knearest_dcell_ptr(xyz.data_ptr(), n, frames, dcell.data_ptr(), dcell.numel(),
+                              k, nn.data_ptr(), d2.data_ptr(), mptr, hint, wait=False)
+    else:
+        ws.knearest_ptr(xyz.data_ptr(), n, frames, c12, k, nn.data_ptr(), d2.data_ptr(),
+                        mptr, hint, wait=False)
+    # the caching allocator must not hand these blocks out again before the
+    # workspace stream is done with them
+    for t in (xyz, nn, d2, m_dev, dcell):
+        if t is not None:
+            t.record_stream(queue)
+    done = torch.cuda.Event()
+    done.record(queue)
+    cur.wait_event(done)
     return nn, d2

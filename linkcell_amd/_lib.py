@@ -31,7 +31,7 @@ EXPORTS = (
     "lc_gpu_sched_collect", "lc_gpu_sched_samples", "lc_gpu_sched_clear",
     "lc_gpu_sched_train", "lc_gpu_sched_get_params", "lc_gpu_sched_set_params",
     "lc_gpu_sched_forward", "lc_gpu_sched_gradient", "lc_gpu_sched_adam",
-    "lc_gpu_last_timing_ex",
+    "lc_gpu_last_timing_ex", "lc_gpu_sched_dump", "lc_gpu_timing_history",
 )
 
 
@@ -104,6 +104,8 @@ _PROTOS = {
     "lc_gpu_sched_tile_count": (C.c_int, [_vp, C.POINTER(_sz)]),
     "lc_gpu_sched_tile_labels": (C.c_int, [_vp, _vp, _sz]),
     "lc_gpu_last_retried": (C.c_int, [_vp, _ip]),
+    "lc_gpu_sched_dump": (C.c_int, [_vp, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lc_gpu_timing_history": (C.c_int, [_vp, _dp, C.c_int, _ip]),
     "lc_gpu_last_timing_ex": (C.c_int, [_vp, _dp, C.c_int]),
     "lc_gpu_sched_collect": (C.c_int, [_vp, C.POINTER(_sz)]),
     "lc_gpu_sched_samples": (C.c_int, [_vp, C.POINTER(_sz)]),

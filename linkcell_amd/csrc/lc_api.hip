@@ -572,6 +572,42 @@ int lc_gpu_sched_tile_labels(lc_gpu_workspace* ws, float* out, size_t n) {
   });
 }
 
+int lc_gpu_timing_history(lc_gpu_workspace* ws, double* ms, int n_runs, int* got) {
+  if (ws == nullptr || ms == nullptr || got == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    const std::vector<lcamd::Timings> h = ws->ws.impl()->engine.timing_history(n_runs);
+    for (size_t i = 0; i < h.size(); ++i) {
+      const lcamd::Timings& t = h[i];
+      const double v[10] = {t.fold, t.scan,  t.scatter, t.sort,  t.walk,
+                            t.exact, t.total, t.setup,   t.pass1, t.retry};
+      for (int j = 0; j < 10; ++j) ms[10 * i + j] = v[j];
+    }
+    *got = static_cast<int>(h.size());
+  });
+}
+
+int lc_gpu_sched_dump(lc_gpu_workspace* ws, size_t n_tiles, size_t n_cells,
+                      unsigned short* x_collect, unsigned short* x_predict,
+                      float* caps, int* tile_cell, double* tile_box, int* count) {
+  if (ws == nullptr || x_collect == nullptr || tile_cell == nullptr ||
+      tile_box == nullptr || count == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    try {
+      ws->ws.impl()->engine.sched_dump(x_collect, x_predict, caps, tile_cell, tile_box,
+                                       count, static_cast<long long>(n_tiles),
+                                       static_cast<long long>(n_cells));
+    } catch (const std::runtime_error& e) {
+      throw linkcell::Error(e.what());
+    }
+  });
+}
+
 int lc_gpu_last_retried(lc_gpu_workspace* ws, int* retried) {
   if (ws == nullptr || retried == nullptr) {
     set_gpu_err("null pointer");

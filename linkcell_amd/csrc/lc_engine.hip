@@ -2704,16 +2704,21 @@ Engine::~Engine() {
   if (graph_staging_ != nullptr) (void)hipHostFree(graph_staging_);
   if (staging_ != nullptr) (void)hipHostFree(staging_);
   if (staging_ev_ != nullptr) (void)hipEventDestroy(staging_ev_);
-  for (auto& e : ev_) {
-    if (e != nullptr) (void)hipEventDestroy(e);
+  for (auto& slot : ring_) {
+    for (auto& e : slot) {
+      if (e != nullptr) (void)hipEventDestroy(e);
+    }
   }
   if (stream_ != nullptr) (void)hipStreamDestroy(stream_);
 }
 
 hipStream_t Engine::stream() {
   if (stream_ == nullptr) {
-    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking),
-          "hipStreamCreate");
+    // default flags, as the reference's cudaStreamCreate
+    // (src/gpu/knearest.cpp:586): the stream is ordered against the null
+    // stream, so a caller's fill or producer kernel there (lc_gpu_fill_i32,
+    // python/src/gpu.rs:293-346) runs before the search that follows it
+    check(hipStreamCreate(&stream_), "hipStreamCreate");
   }
   return stream_;
 }
@@ -2755,24 +2760,35 @@ void Engine::free_all() {
   }
 }
 
+namespace {
+
+Timings timings_of(hipEvent_t* ev) {
+  auto el = [&](int a, int b) {
+    float ms = 0;
+    check(hipEventElapsedTime(&ms, ev[a], ev[b]), "elapsed");
+    return ms;
+  };
+  Timings t;
+  t.fold = el(0, 1);
+  t.scan = el(1, 2);
+  t.scatter = el(2, 3);
+  t.sort = el(3, 4);
+  t.walk = el(4, 5);
+  t.exact = el(5, 6);
+  t.total = el(0, 7);
+  t.setup = el(4, 8);
+  t.pass1 = el(8, 9);
+  t.retry = el(9, 5);
+  return t;
+}
+
+}  // namespace
+
 const Timings& Engine::last_timings() {
   if (timings_pending_) {
-    check(hipEventSynchronize(ev_[7]), "event wait");
-    auto el = [&](int a, int b) {
-      float ms = 0;
-      check(hipEventElapsedTime(&ms, ev_[a], ev_[b]), "elapsed");
-      return ms;
-    };
-    timings_.fold = el(0, 1);
-    timings_.scan = el(1, 2);
-    timings_.scatter = el(2, 3);
-    timings_.sort = el(3, 4);
-    timings_.walk = el(4, 5);
-    timings_.exact = el(5, 6);
-    timings_.total = el(0, 7);
-    timings_.setup = el(4, 8);
-    timings_.pass1 = el(8, 9);
-    timings_.retry = el(9, 5);
+    hipEvent_t* ev = ring_[(ring_pos_ - 1) % kTimingRing];
+    check(hipEventSynchronize(ev[7]), "event wait");
+    timings_ = timings_of(ev);
     int dc = 0;
     check(hipMemcpy(&dc, as<int>(counters_.p) + 1, sizeof(int),
                     hipMemcpyDeviceToHost),
@@ -2786,6 +2802,17 @@ const Timings& Engine::last_timings() {
     timings_pending_ = false;
   }
   return timings_;
+}
+
+std::vector<Timings> Engine::timing_history(int n) {
+  const Timings last = last_timings();
+  const long long have = std::min<long long>(ring_pos_, kTimingRing);
+  const long long m = std::min<long long>(std::max(n, 0), have);
+  std::vector<Timings> out;
+  for (long long i = ring_pos_ - m; i < ring_pos_; ++i) {
+    out.push_back(i == ring_pos_ - 1 ? last : timings_of(ring_[i % kTimingRing]));
+  }
+  return out;
 }
 
 void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
@@ -2924,8 +2951,11 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   const bool timing = timing_ && !capturing_;
 
   if (timing) {
-    for (auto& e : ev_) {
-      if (e == nullptr) check(hipEventCreate(&e), "hipEventCreate");
+    // a slot comes round again kTimingRing timed runs later (recording an
+    // event again while its last record is pending is allowed)
+    ev_ = ring_[ring_pos_ % kTimingRing];
+    for (int e = 0; e < 10; ++e) {
+      if (ev_[e] == nullptr) check(hipEventCreate(&ev_[e]), "hipEventCreate");
     }
     check(hipEventRecord(ev_[0], st), "event record");
   }
@@ -3111,6 +3141,7 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   if (timing) {
     check(hipEventRecord(ev_[6], st), "event record");
     check(hipEventRecord(ev_[7], st), "event record");
+    ring_pos_ += 1;
     timings_pending_ = true;
   }
   if (wait && !capturing_) sync();
@@ -3219,6 +3250,53 @@ long long Engine::sched_collect() {
   tile_labels(as<float>(labels_.p), nt);
   return predictor().collect(tile_src(last_ctot_, last_F_), nt,
                              as<float>(labels_.p), stream());
+}
+
+void Engine::sched_dump(unsigned short* x_collect, unsigned short* x_predict,
+                        float* caps, int* tile_cell, double* tile_box, int* count,
+                        long long nt_cap, long long nc_cap) {
+  const long long nt = tile_count();
+  if (nt_cap < nt || nc_cap < last_ctot_) {
+    throw std::runtime_error("dump buffers shorter than the search's tiles or cells");
+  }
+  if (nt == 0) return;
+  hipStream_t st = stream();
+  const TileSrc src = tile_src(last_ctot_, last_F_);
+  const size_t xb = sizeof(unsigned short) * static_cast<size_t>(nt) * kFeat;
+  void* dx = nullptr;
+  void* dp = nullptr;
+  void* dc = nullptr;
+  check(hipMalloc(&dx, xb), "hipMalloc");
+  check(hipMalloc(&dp, xb + sizeof(float) * static_cast<size_t>(nt)), "hipMalloc");
+  try {
+    predictor().collect_features(src, nt, static_cast<unsigned short*>(dx), st);
+    check(hipMemcpyAsync(x_collect, dx, xb, hipMemcpyDeviceToHost, st), "DtoH features");
+    if (predictor().trained() && x_predict != nullptr && caps != nullptr) {
+      dc = static_cast<char*>(dp) + xb;
+      predictor().predict_caps(src, nt, static_cast<float*>(dc), st,
+                               static_cast<unsigned short*>(dp));
+      check(hipMemcpyAsync(x_predict, dp, xb, hipMemcpyDeviceToHost, st), "DtoH features");
+      check(hipMemcpyAsync(caps, dc, sizeof(float) * static_cast<size_t>(nt),
+                           hipMemcpyDeviceToHost, st),
+            "DtoH caps");
+    }
+    check(hipMemcpyAsync(tile_cell, tile_cell_.p, sizeof(int) * static_cast<size_t>(nt),
+                         hipMemcpyDeviceToHost, st),
+          "DtoH tile cells");
+    check(hipMemcpyAsync(tile_box, tbox_.p, sizeof(Box) * static_cast<size_t>(nt),
+                         hipMemcpyDeviceToHost, st),
+          "DtoH tile boxes");
+    check(hipMemcpyAsync(count, count_.p, sizeof(int) * static_cast<size_t>(last_ctot_),
+                         hipMemcpyDeviceToHost, st),
+          "DtoH counts");
+    check(hipStreamSynchronize(st), "stream wait");
+  } catch (...) {
+    (void)hipFree(dx);
+    (void)hipFree(dp);
+    throw;
+  }
+  check(hipFree(dx), "hipFree");
+  check(hipFree(dp), "hipFree");
 }
 
 long long Engine::tile_count() {

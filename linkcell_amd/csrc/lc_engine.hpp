@@ -115,6 +115,10 @@ class Engine {
 
   void set_timing(bool on) { timing_ = on; }
   const Timings& last_timings();
+  // Stage timings of up to n of the most recent timed runs, oldest first
+  // (at most kTimingRing); waits for the latest. Deferred and retried
+  // counts are those of the last run only.
+  std::vector<Timings> timing_history(int n);
   void set_mode(WalkMode m) { mode_ = m; }
   // Home cells with at least this many sources take the wave-per-cell
   // walk, the rest the lane-per-source walk.
@@ -154,6 +158,14 @@ class Engine {
   // Append the source tiles of the last search (run with out_d2) to the
   // predictor's training set; returns the samples added.
   long long sched_collect();
+  // Diagnostics of the learned scheduler on the last search (host
+  // arrays, nt >= tile_count(), nc >= the search's cells): the inputs the
+  // training set takes for every tile, the inputs and caps of inference
+  // (when trained), and the tile layout they come from: home cell, box
+  // (lx, hx, ly, hy, lz, hz, pad, pad) and cell occupancy.
+  void sched_dump(unsigned short* x_collect, unsigned short* x_predict, float* caps,
+                  int* tile_cell, double* tile_box, int* count, long long nt,
+                  long long nc);
   // Device work counters of the walks (diagnostics).
   void set_stats(bool on) { stats_on_ = on; }
   void read_stats(unsigned long long* out, int n);
@@ -197,7 +209,12 @@ class Engine {
   std::vector<std::pair<std::string, int>> tuned_;
   bool stats_on_ = false;
   long long stats_n_ = 0;
-  hipEvent_t ev_[10] = {};
+  // timing events of the last kTimingRing timed runs (slot ring_pos_ - 1
+  // is the latest), so a timed loop of many calls can be read afterwards
+  static constexpr int kTimingRing = 64;
+  hipEvent_t ring_[kTimingRing][10] = {};
+  long long ring_pos_ = 0;
+  hipEvent_t* ev_ = nullptr;  // the events of the run being recorded
   Timings timings_{};
 };
 

@@ -570,7 +570,8 @@ __global__ __launch_bounds__(256) void k_mlp_forward(
 // forward pass, out[T] = cell_min * 2^(y + margin).
 __global__ __launch_bounds__(256) void k_mlp_predict(Net N, TileSrc S,
                                                      float margin,
-                                                     float* __restrict__ out) {
+                                                     float* __restrict__ out,
+                                                     unsigned short* __restrict__ xdump) {
   __shared__ __align__(16) unsigned short sX[kMlpRows * kXS];
   __shared__ __align__(16) unsigned short sA1[kMlpRows * kHS];
   __shared__ float sCM[kMlpRows];
@@ -624,6 +625,13 @@ __global__ __launch_bounds__(256) void k_mlp_predict(Net N, TileSrc S,
     }
   }
   __syncthreads();
+  if (xdump != nullptr) {
+    // diagnostics: the inputs exactly as the forward pass reads them
+    for (int e = threadIdx.x; e < kMlpRows * kFeat; e += blockDim.x) {
+      const long long T = t0 + e / kFeat;
+      if (T < ntiles) xdump[T * kFeat + e % kFeat] = sX[(e / kFeat) * kXS + e % kFeat];
+    }
+  }
   f32x4 z1[4], z2[4];
   float y[4];
   fwd_wave(N, sX, sA1, 16 * wv, lane, z1, z2, y);
@@ -657,7 +665,7 @@ __global__ __launch_bounds__(256) void k_sched_collect(
     v[s] = lo | (hi << 16);
   }
   reinterpret_cast<uint4*>(X + T * kFeat)[q] = make_uint4(v[0], v[1], v[2], v[3]);
-  if (q == 0) {
+  if (q == 0 && labels != nullptr) {
     const float lab = labels[T];
     const bool fin = lab < __builtin_inff();
     Tg[T] = fin ? log2f(lab / static_cast<float>(c.cell_min)) : 0.0f;
@@ -857,6 +865,14 @@ long long Predictor::collect(const TileSrc& src, long long ntiles,
   return ntiles;
 }
 
+void Predictor::collect_features(const TileSrc& src, long long ntiles,
+                                 unsigned short* x, hipStream_t st) {
+  if (ntiles <= 0) return;
+  k_sched_collect<<<blocks_for(ntiles * (kFeat / 8), 256), 256, 0, st>>>(
+      src, ntiles, nullptr, x, nullptr, nullptr);
+  check(hipGetLastError(), "collect launch");
+}
+
 double Predictor::train(int epochs, int batch, double lr, unsigned seed,
                         double quantile, hipStream_t st) {
   if (n_samples_ < kMlpRows) throw std::runtime_error("too few training samples");
@@ -941,14 +957,14 @@ double Predictor::train(int epochs, int batch, double lr, unsigned seed,
 }
 
 void Predictor::predict_caps(const TileSrc& src, long long tcap, float* out,
-                             hipStream_t st) {
+                             hipStream_t st, unsigned short* xdump) {
   if (!trained_) throw std::runtime_error("the predictor is not trained");
   Net N{static_cast<const unsigned short*>(w1t_.p),
         static_cast<const unsigned short*>(w2t_.p),
         static_cast<const unsigned short*>(w2_.p),
         static_cast<const float*>(master_.p)};
   k_mlp_predict<<<blocks_for(tcap, kMlpRows), 256, 0, st>>>(
-      N, src, static_cast<float>(margin_), out);
+      N, src, static_cast<float>(margin_), out, xdump);
   check(hipGetLastError(), "mlp predict launch");
 }
 

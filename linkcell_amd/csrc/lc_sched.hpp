@@ -54,8 +54,10 @@ class Predictor {
   // Learned caps of every tile of a search: out[T] = cell_min *
   // 2^(y_T + margin) for T below the search's tile count (read on the
   // device), tcap the caller's upper bound of that count.
+  // xdump (optional, device): the BF16 inputs of every tile as the
+  // forward pass reads them (diagnostics).
   void predict_caps(const TileSrc& src, long long tcap, float* out,
-                    hipStream_t st);
+                    hipStream_t st, unsigned short* xdump = nullptr);
 
   // Append the tiles of a finished search with a finite label to the
   // training set (labels: device floats per tile, from the engine's
@@ -63,6 +65,10 @@ class Predictor {
   long long collect(const TileSrc& src, long long ntiles, const float* labels,
                     hipStream_t st);
   long long samples() const { return n_samples_; }
+  // The BF16 inputs the training set would take for every tile of a
+  // search, into the device array x (ntiles x kFeat; diagnostics).
+  void collect_features(const TileSrc& src, long long ntiles, unsigned short* x,
+                        hipStream_t st);
   void clear_samples();
 
   // Fresh initialisation from seed, then epochs of minibatch Adam over the

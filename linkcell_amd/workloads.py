@@ -210,3 +210,20 @@ def fcc_c5ii(cells=128, a=1.5625, jitter=0.15, seed=5500):
     L = cells * a
     return dict(name=f"fcc-{4 * cells ** 3}-k32", xyz=np.ascontiguousarray(xyz),
                 cell=np.array([L, L, L]), k=32, hint=3.0, mask=None)
+
+
+def fcc_c5ii_device(seed, device, cells=128, a=1.5625, jitter=0.15):
+    """The jittered FCC lattice of fcc_c5ii drawn on the device with torch's
+    generator (same lattice, fresh jitter per seed): the learned scheduler's
+    training frames for the 8M-point FCC workload."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    base = torch.tensor([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]],
+                        dtype=torch.float64, device=device)
+    r = torch.arange(cells, dtype=torch.float64, device=device)
+    gx, gy, gz = torch.meshgrid(r, r, r, indexing="ij")
+    grid = torch.stack([gx.reshape(-1), gy.reshape(-1), gz.reshape(-1)], dim=1)
+    sites = (grid[:, None, :] + base[None, :, :]).reshape(-1, 3) * a
+    noise = torch.randn(sites.shape, dtype=torch.float64, device=device, generator=g)
+    return (sites + noise * jitter).contiguous()

@@ -283,6 +283,20 @@ typedef struct kheap {
   size_t n, k;
 } kheap;
 
+/* Planted mistakes, for tests/test_oracle.py only: each variant is a
+ * plausible misreading of KHeap's tie rules, compiled into a separate
+ * library by the test to show that the hand-derived tie fixtures
+ * (tests/golden/tie_answers.json) reject it. The oracle itself is built
+ * with LC_ORACLE_PLANT 0 (none). */
+#ifndef LC_ORACLE_PLANT
+#define LC_ORACLE_PLANT 0
+#endif
+#define LC_PLANT_REPLACE_ON_EQUAL 1   /* d2 <= worst replaces */
+#define LC_PLANT_LAST_ARGMAX 2        /* the last of equal maxima is the worst */
+#define LC_PLANT_CANONICAL_INDEX 3    /* ties retained by smallest index */
+#define LC_PLANT_INCREASING_LISTS 4   /* cell lists in increasing index */
+#define LC_PLANT_FINISH_DESCENDING 5  /* finish orders equal d2 by larger index */
+
 /* KHeap::push, src/knearest.rs:106-129 */
 static void kheap_push(kheap *h, double d2, size_t j) {
   for (size_t t = 0; t < h->n; ++t) {
@@ -301,9 +315,24 @@ static void kheap_push(kheap *h, double d2, size_t j) {
     return;
   }
   size_t worst = 0;
-  for (size_t t = 1; t < h->n; ++t)
+  for (size_t t = 1; t < h->n; ++t) {
+#if LC_ORACLE_PLANT == LC_PLANT_LAST_ARGMAX
+    if (h->d2[t] >= h->d2[worst]) worst = t;
+#elif LC_ORACLE_PLANT == LC_PLANT_CANONICAL_INDEX
+    if (h->d2[t] > h->d2[worst] ||
+        (h->d2[t] == h->d2[worst] && h->idx[t] > h->idx[worst]))
+      worst = t;
+#else
     if (h->d2[t] > h->d2[worst]) worst = t;
+#endif
+  }
+#if LC_ORACLE_PLANT == LC_PLANT_REPLACE_ON_EQUAL
+  if (d2 <= h->d2[worst]) {
+#elif LC_ORACLE_PLANT == LC_PLANT_CANONICAL_INDEX
+  if (d2 < h->d2[worst] || (d2 == h->d2[worst] && j < h->idx[worst])) {
+#else
   if (d2 < h->d2[worst]) {
+#endif
     h->d2[worst] = d2;
     h->idx[worst] = j;
   }
@@ -336,7 +365,11 @@ static void kheap_finish(kheap *h) {
     size_t p = a;
     while (p > 0) {
       int64_t kp = total_key(h->d2[p - 1]);
+#if LC_ORACLE_PLANT == LC_PLANT_FINISH_DESCENDING
+      if (kp > kv || (kp == kv && h->idx[p - 1] < jv)) {
+#else
       if (kp > kv || (kp == kv && h->idx[p - 1] > jv)) {
+#endif
         h->d2[p] = h->d2[p - 1];
         h->idx[p] = h->idx[p - 1];
         --p;
@@ -494,7 +527,11 @@ static size_t *prepare_frame(frame_ctx *f, const double *xyz,
     orc_fold_bin(f->cell, &f->plan, xyz + 3 * i, f->folded[i], f->bin[i]);
   }
   for (size_t t = 0; t < na; ++t) {
+#if LC_ORACLE_PLANT == LC_PLANT_INCREASING_LISTS
+    size_t i = active[na - 1 - t];
+#else
     size_t i = active[t];
+#endif
     int64_t c = cell_index(f, f->bin[i][0], f->bin[i][1], f->bin[i][2]);
     f->next[i] = f->head[c];
     f->head[c] = (int64_t)i;

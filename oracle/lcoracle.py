@@ -45,15 +45,27 @@ class _Plan(C.Structure):
 
 
 _lib = None
+_variants = {}
 
 
-def load():
+def load(path=None):
+    """The oracle library; `path` loads another build of the same source
+    (the tie tests' planted-mistake variants)."""
     global _lib
+    if path is not None:
+        path = str(path)
+        if path not in _variants:
+            _variants[path] = _bind(C.CDLL(path))
+        return _variants[path]
     if _lib is not None:
         return _lib
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} missing; run `make -C oracle`")
-    lib = C.CDLL(str(LIB_PATH))
+    _lib = _bind(C.CDLL(str(LIB_PATH)))
+    return _lib
+
+
+def _bind(lib):
     vp, dp, sz = C.c_void_p, C.POINTER(C.c_double), C.c_size_t
     lib.orc_cell_from_vectors.argtypes = [dp, dp, dp, dp, C.POINTER(_Cell)]
     lib.orc_cell_from_vectors.restype = C.c_int
@@ -77,7 +89,6 @@ def load():
     lib.orc_knearest_brute.argtypes = [vp, sz, dp, sz, vp, vp, vp]
     lib.orc_knearest_brute.restype = C.c_int
     lib.orc_version.restype = C.c_char_p
-    _lib = lib
     return lib
 
 
@@ -167,10 +178,10 @@ def _threads(nthreads):
 
 
 def knearest_many(xyz, cell, k, mask=None, cell_hint=0.0, n_frames=None,
-                  nthreads=1, stats=False):
+                  nthreads=1, stats=False, lib_path=None):
     """lc_knearest_many restated. xyz: (n, 3) or (F, n, 3) float64.
     Returns (nn, d2) shaped (F*n, k) [plus (reach, cand) when stats]."""
-    lib = load()
+    lib = load(lib_path)
     x = np.ascontiguousarray(xyz, dtype=np.float64)
     if x.ndim == 3:
         F, n = x.shape[0], x.shape[1]

@@ -10,6 +10,18 @@ known_answers.json
     in the reference). "brute" marks cases the reference also checks
     against knearest_brute / the 27-image oracle.
 
+tie_answers.json
+    Hand-derived rows for exact d2 ties at and inside the k-th slot, the
+    one part of KHeap (src/knearest.rs:106-158) that random inputs never
+    reach. Every coordinate is a multiple of 0.5 in a box of edge 16 or 32
+    (fold, shift and d2 are then exact), so each asserted row follows from
+    the visit order alone: stencil cells lexicographic in (dx, dy, dz)
+    (:436-438), each cell's list in decreasing index (:423-428), pushes
+    kept while the heap has room, then the first argmax replaced on a
+    strict < (:120-128), finish by (d2, index) (:146-158). The derivation
+    of every row is written out below and stored with the case. None of
+    these answers comes from the oracle.
+
 c1_uniform_oracle.npz
     The oracle's full output for the N=4096 uniform 16^3 box, k=12, hint
     3.0 (LCG seed 1): int32 indices and the float64 d2 bit patterns as
@@ -141,6 +153,73 @@ def cases():
     return out
 
 
+# Box of edge 16, hint 4.0: widths 16 (exact), mesh 4 x 4 x 4 of edge 4,
+# cell_min 4, max_reach 3; s = x / 16 and 16 s are exact, bin = floor(x / 4).
+# A source at (4, 4, 4) sits in cell (1, 1, 1); its shell-1 cells 0..2 per
+# axis need no wrap, and cell offset (dx, dy, dz) is visited at position
+# P = 9 (dx + 1) + 3 (dy + 1) + (dz + 1) (home = 13).
+B16 = [16.0, 16.0, 16.0]
+B32 = [32.0, 32.0, 32.0]
+
+
+def tie_cases():
+    out = []
+
+    def add(name, cell, hint, k, xyz, rows, d2, derivation, plants):
+        out.append(dict(name=name, cell=cell, hint=hint, k=k, xyz=xyz, rows=rows,
+                        d2=d2, derivation=derivation, fails_if=plants))
+
+    add("tie_across_cells_first_visit_kept", B16, 4.0, 2,
+        [[4, 4, 4], [4, 4, 3], [4, 3, 4], [3, 4, 4]],
+        {"0": [2, 3]}, {"0": [1.0, 1.0]},
+        "source 0 in cell (1,1,1); 3 in (0,1,1) at P4, 2 in (1,0,1) at P10, 1 in "
+        "(1,1,0) at P12, each at d2 1. Pushes 3, 2 fill the heap; 1 meets the first "
+        "argmax (slot 0, d2 1) and 1 < 1 fails, so it is dropped. Stop at reach 1 "
+        "(1 <= 16). finish: (1,2), (1,3).",
+        ["replace_on_equal", "canonical_index"])
+    add("tie_in_one_cell_decreasing_index_kept", B16, 4.0, 2,
+        [[2, 2, 2], [1, 2, 2], [3, 2, 2], [2, 1, 2]],
+        {"0": [2, 3]}, {"0": [1.0, 1.0]},
+        "all four points in cell (0,0,0); the only occupied stencil cell of source 0 "
+        "is its home, whose list runs 3, 2, 1, 0. Pushes 3, 2 fill the heap, 1 (d2 1) "
+        "is not < 1 and is dropped. finish: (1,2), (1,3).",
+        ["replace_on_equal", "canonical_index", "increasing_lists"])
+    add("tie_eviction_takes_first_argmax", B16, 4.0, 2,
+        [[4, 4, 4], [2, 4, 4], [4, 2, 4], [4, 4, 3], [6, 4, 4]],
+        {"0": [3, 2]}, {"0": [1.0, 4.0]},
+        "visit order for source 0: 1 (P4, d2 4) -> slot 0, 2 (P10, d2 4) -> slot 1, "
+        "3 (P12, d2 1): first argmax of [4, 4] is slot 0, 1 < 4, slot 0 := 3; "
+        "4 (home P13, d2 4): argmax of [1, 4] is slot 1, 4 < 4 fails. Heap {3, 2}. "
+        "finish: (1,3), (4,2).",
+        ["replace_on_equal", "canonical_index", "last_argmax"])
+    add("tie_eviction_by_slot_not_age", B16, 4.0, 2,
+        [[4, 4, 4], [1, 4, 4], [4, 4, 2], [4, 2, 4], [5, 4, 4]],
+        {"0": [4, 3]}, {"0": [1.0, 4.0]},
+        "visit order for source 0: 1 (P4, d2 9) -> slot 0, 3 (P10, d2 4) -> slot 1, "
+        "2 (P12, d2 4): argmax of [9, 4] is slot 0, slot 0 := 2 (heap [2:4, 3:4]); "
+        "4 (home P13, d2 1): first argmax of [4, 4] is slot 0, slot 0 := 4, so the "
+        "newer tied entry (2) leaves and the older (3) stays. finish: (1,4), (4,3).",
+        ["canonical_index", "last_argmax"])
+    add("tie_across_shells_inner_shell_kept", B32, 4.0, 2,
+        [[4, 4, 4], [4, 4, 5], [31.5, 4, 4], [4, 4, 8.5]],
+        {"0": [1, 3]}, {"0": [1.0, 20.25]},
+        "box 32, hint 4: mesh 8 x 8 x 8, cell_min 4, max_reach 5. Source 0 in cell "
+        "(1,1,1). Shell 1: home list 1, 0 pushes 1 (d2 1); (0,0,1) = cell (1,1,2) "
+        "pushes 3 (d2 4.5^2 = 20.25). Full, but 20.25 > (1*4)^2: walk shell 2. Point "
+        "2 (cell 7) is reached only at jx = -1 (offset dx = -2) with shift -32: "
+        "((31.5 - 32) - 4)^2 = 20.25, not < 20.25, dropped. Stop at reach 2 "
+        "(20.25 <= 64). finish: (1,1), (20.25,3).",
+        ["replace_on_equal", "canonical_index"])
+    add("equal_d2_inside_row_sorted_by_index", B16, 4.0, 3,
+        [[4, 4, 4], [6, 4, 4], [4, 4, 5], [3, 4, 4]],
+        {"0": [2, 3, 1]}, {"0": [1.0, 1.0, 4.0]},
+        "three candidates for k = 3, all kept: 3 (P4, d2 1), then the home list 3, 2, "
+        "1, 0 gives 2 (d2 1) and 1 (d2 4). finish sorts by (d2, index): (1,2), (1,3), "
+        "(4,1).",
+        ["finish_descending_index"])
+    return out
+
+
 def main():
     from oracle import lcoracle as orc
     from linkcell_amd import workloads as wl
@@ -150,6 +229,10 @@ def main():
     (here / "known_answers.json").write_text(json.dumps(
         {"source": "linkcell 0.3.3 reference tests and doc tests", "cases": cs},
         indent=1))
+    ties = tie_cases()
+    (here / "tie_answers.json").write_text(json.dumps(
+        {"source": "hand-derived from src/knearest.rs:106-158, :423-438 (see "
+                   "tests/golden/make_golden.py)", "cases": ties}, indent=1))
     c1 = wl.uniform_c1()
     nn, d2, reach, cand = orc.knearest_many(c1["xyz"], c1["cell"], c1["k"],
                                             cell_hint=c1["hint"], stats=True)

@@ -233,6 +233,47 @@ def test_python_knearest_host_and_device(gpu):
     x3 = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], dtype=np.float64)
     mn, _ = lc.knearest(x3, cell, 1, mask=m)
     assert mn[:, 0].tolist() == [2, -1, 0]
+    # device masks follow the host rules (python/src/lib.rs:188-211): int32
+    # as given, float64 nonzero keeps, anything else raises
+    dx3 = torch.from_numpy(x3).cuda()
+    dn, _ = lc.knearest(dx3, cell, 1, mask=torch.from_numpy(m).cuda())
+    assert dn[:, 0].tolist() == [2, -1, 0]
+    dn, _ = lc.knearest(dx3, cell, 1, mask=torch.tensor([1.0, 0.0, 2.5]).double().cuda())
+    assert dn[:, 0].tolist() == [2, -1, 0]
+    with pytest.raises(ValueError, match="int32 or float64"):
+        lc.knearest(dx3, cell, 1, mask=torch.tensor([1, 0, 1], dtype=torch.int64).cuda())
+    with pytest.raises(ValueError, match="int32 or float64"):
+        lc.knearest(x3, cell, 1, mask=np.array([1, 0, 1], dtype=np.int64))
+    # a device cell must be float64, as the reference binding requires
+    with pytest.raises(ValueError, match="float64"):
+        lc.knearest(dx3, torch.tensor([10.0, 10.0, 10.0], dtype=torch.float32).cuda(), 1)
+    # the search is ordered on torch's current stream without a host wait:
+    # a producer kernel right before it and a consumer right after it see
+    # the right data
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        big = torch.rand((20000, 3), dtype=torch.float64, device="cuda") * 20.0
+        big.mul_(1.0)
+        bn1, bd1 = lc.knearest(big, [20.0, 20.0, 20.0], 6)
+        cn, cd = bn1.clone(), bd1.clone()
+    s.synchronize()
+    hn, hd = lc.knearest(big.cpu().numpy(), [20.0, 20.0, 20.0], 6)
+    assert np.array_equal(cn.cpu().numpy(), hn)
+    assert same_bits(cd.cpu().numpy(), hd)
+    # one workspace per thread
+    import threading
+    seen = []
+
+    def worker():
+        seen.append(lc._workspace())  # kept alive past the thread's end
+
+    th = [threading.Thread(target=worker) for _ in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    mine = lc._workspace()
+    assert seen[0] is not seen[1] and all(w is not mine for w in seen)
 
 
 @pytest.mark.parametrize("shape", ["3", "9", "12"])
@@ -292,3 +333,58 @@ def test_non_finite_coordinates_follow_the_oracle(ws, k):
         assert np.array_equal(nn[i][gf], on[i][of]), i
         assert same_bits(d2[i][gf], od[i][of]), i
         assert sorted(nn[i][~gf].tolist()) == sorted(on[i][~of].tolist()), i
+
+
+TIES = json.loads((GOLD / "tie_answers.json").read_text())["cases"]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("cs", TIES, ids=[c["name"] for c in TIES])
+def test_tie_answers_on_device(ws, cs, mode):
+    """The hand-derived tie rows (tests/golden/make_golden.py) on the GPU:
+    the fast walk must see each boundary tie and hand the source to the
+    exact-order walk (mode 0), or run it there directly (mode 1)."""
+    ws.set_mode(mode)
+    ws.set_timing(True)
+    try:
+        nn, d2, reach = run_device(ws, cs["xyz"], cs["cell"], cs["k"], None, cs["hint"],
+                                   reach=True)
+        deferred = ws.last_timing()["deferred"]
+    finally:
+        ws.set_mode(0)
+        ws.set_timing(False)
+    for r, want in cs["rows"].items():
+        assert nn[int(r)].tolist() == want, cs["derivation"]
+        assert d2[int(r)].tolist() == cs["d2"][r], cs["derivation"]
+    on, od, orr, _ = oracle(cs["xyz"], cs["cell"], cs["k"], None, cs["hint"], stats=True)
+    assert np.array_equal(nn, on) and same_bits(d2, od)
+    assert np.array_equal(reach, orr)
+    if mode == 0 and cs["name"].startswith("tie_"):
+        assert deferred >= 1
+
+
+@pytest.mark.parametrize("k", [129, 200])
+def test_exact_walk_beyond_wave_heaps(ws, k):
+    """k above 128 runs the lane-per-source exact walk with its heap in HBM
+    scratch: a small box whose stencil wraps (several images of one point),
+    a masked frame, coincident points; indices, d2 bits and reach."""
+    rng = np.random.default_rng(k)
+    cases = []
+    # small box, 300 points: k exceeds the other points for no source, the
+    # stencil wraps (mesh 2 per axis at hint 3)
+    cases.append((rng.random((300, 3)) * 7.0, [7.0, 7.0, 7.0], None, 3.0))
+    # masked triclinic frame with coincident points
+    H = np.array([[11.0, 0, 0], [2.0, 10.0, 0], [-1.0, 1.5, 9.0]])
+    base = (rng.random((420, 3)) * 1.1 - 0.05) @ H
+    xyz = np.concatenate([base, base[:60]])
+    mask = (rng.random(len(xyz)) > 0.15).astype(np.int32)
+    cases.append((xyz, H, mask, 2.5))
+    # fewer active points than k: rows end in -1 / NaN
+    cases.append((rng.random((150, 3)) * 9.0, [9.0, 9.0, 9.0], None, 2.0))
+    for xyz, cell, m, hint in cases:
+        nn, d2, reach = run_device(ws, xyz, cell, k, m, hint, reach=True)
+        on, od, orr, _ = oracle(xyz, cell, k, m, hint, stats=True)
+        assert np.array_equal(nn, on)
+        assert same_bits(d2, od)
+        assert np.array_equal(reach, orr)
+    assert (nn[:, 149:] == -1).all()

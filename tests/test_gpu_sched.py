@@ -273,3 +273,99 @@ def test_labels_are_the_rows_largest_kth_distance(ws):
     ups = np.unique(np.array([f32_up(v) for v in np.unique(kth)]))
     assert np.isin(L, ups).all()
     assert len(L) >= s.n // 16
+
+
+def _bf16_bits(x64):
+    """BF16 bit pattern (round to nearest even through FP32) of float64 values."""
+    return (mlp.bf16(x64).astype(np.float32).view(np.uint32) >> 16).astype(np.uint16)
+
+
+def _bf16_ambiguous(x64, ulps=8):
+    """True where an FP32 evaluation within `ulps` FP32 ulps of x could round
+    to either BF16 neighbour (x sits that close to a BF16 rounding midpoint)."""
+    x = np.asarray(x64, dtype=np.float64)
+    lo = mlp.bf16(x * (1.0 - ulps * 2.0 ** -23))
+    hi = mlp.bf16(x * (1.0 + ulps * 2.0 ** -23))
+    return lo != hi
+
+
+def test_feature_maps_match_between_training_inference_and_oracle(ws):
+    """The 160 inputs of every tile of a clustered search, as training
+    collects them and as inference reads them, are the same BF16 values;
+    both equal mlp.tile_features on the tile layout (exactly, except where
+    an FP32 log2 lies within rounding of a BF16 midpoint); the cell
+    occupancies equal a numpy bin count; inference's caps follow
+    mlp.forward on the dumped inputs."""
+    cfg = wl.clustered_c3_small(5)
+    s = _Search(ws, cfg)
+    s.run()
+    ws.sched_clear()
+    ws.sched_collect()
+    ws.sched_train(epochs=2, batch=4096, lr=4e-3, seed=1, quantile=0.999)
+    p, margin = ws.sched_get_params()
+    s.run()
+    L = np.asarray(cfg["cell"], dtype=np.float64)
+    oc = orc.Cell.ortho(*L)
+    plan = oc.plan(cfg["hint"])
+    nx, ny, nz = plan["nx"], plan["ny"], plan["nz"]
+    cm = plan["cell_min"]
+    d = ws.sched_dump(nx * ny * nz)
+    xc, xp = d["x_collect"], d["x_predict"]
+    # 1. training and inference read identical inputs
+    assert np.array_equal(xc, xp)
+    # 2. occupancies: the fold and bin of src/cell.rs:176-194 and
+    #    src/knearest.rs:405-412 in numpy
+    sf = cfg["xyz"] / np.array(oc.widths)
+    sf = sf - np.floor(sf)
+    sf = np.where(sf >= 1.0, 0.0, sf)
+    nvec = np.array([nx, ny, nz])
+    b = np.clip((sf * nvec).astype(np.int64), 0, nvec - 1)
+    cid = (b[:, 2] * ny + b[:, 1]) * nx + b[:, 0]
+    count = np.bincount(cid, minlength=nx * ny * nz)
+    assert np.array_equal(d["count"], count)
+    # 3. the CPU feature map on the device's tile layout
+    cells, box = d["tile_cell"], d["tile_box"]
+    nt = len(cells)
+    assert nt == np.sum((count + 15) // 16)
+    first = np.concatenate([[0], np.cumsum((count + 15) // 16)])
+    tl = np.arange(nt) - first[cells]
+    fill = np.minimum(16, count[cells] - 16 * tl)
+    assert (fill >= 1).all()
+    c3 = count.reshape(nz, ny, nx)
+    hx, hy, hz = cells % nx, (cells // nx) % ny, cells // (nx * ny)
+    ext = np.stack([(box[:, 1] - box[:, 0]) / cm, (box[:, 3] - box[:, 2]) / cm,
+                    (box[:, 5] - box[:, 4]) / cm], axis=1).astype(np.float32)
+    hinv = np.array(oc.hinv)
+    mid = np.stack([0.5 * (box[:, 0] + box[:, 1]), 0.5 * (box[:, 2] + box[:, 3]),
+                    0.5 * (box[:, 4] + box[:, 5])], axis=1)
+    sfrac = np.stack([(hinv[0] * mid[:, 0] + hinv[3] * mid[:, 1]) + hinv[6] * mid[:, 2],
+                      (hinv[1] * mid[:, 0] + hinv[4] * mid[:, 1]) + hinv[7] * mid[:, 2],
+                      (hinv[2] * mid[:, 0] + hinv[5] * mid[:, 1]) + hinv[8] * mid[:, 2]],
+                     axis=1)
+    pos = (sfrac * nvec - np.stack([hx, hy, hz], axis=1)).astype(np.float32)
+    want = np.empty((nt, mlp.FEAT))
+    for t in range(nt):
+        want[t] = mlp.tile_features(c3, (nx, ny, nz), (hx[t], hy[t], hz[t]),
+                                    ext[t].astype(np.float64), pos[t].astype(np.float64),
+                                    fill[t], cfg["k"])
+    wb = _bf16_bits(want)
+    amb = _bf16_ambiguous(want)
+    # position, fill and k inputs are exact casts, never ambiguous
+    amb[:, 128:] = False
+    same = wb == xc
+    assert same[~amb].all(), np.argwhere(~same & ~amb)[:5]
+    near = (np.abs(wb.astype(np.int64) - xc.astype(np.int64)) <= 1)
+    assert near[amb].all()
+    assert amb.mean() < 2e-3
+    # 4. inference's caps: cell_min * 2^(y + margin), y = mlp.forward on the
+    #    dumped inputs. All but a few tiles within the tolerance of
+    #    test_forward_matches_oracle; the rest differ by one BF16 rounding
+    #    of a hidden activation, which an FP32 sum one ulp away from the
+    #    oracle's FP64 sum can flip (a 2^-8 relative step in one input of
+    #    the second layer)
+    Xd = (xc.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    y = mlp.forward(p.astype(np.float64), Xd, emulate=True)["y"]
+    y_dev = np.log2(d["caps"].astype(np.float64) / np.float32(cm)) - margin
+    err = np.abs(y_dev - y)
+    assert np.quantile(err, 0.999) <= 1e-4 * (1.0 + np.max(np.abs(y))) + 1e-5
+    assert err.max() <= 2.0 ** -6

@@ -271,3 +271,53 @@ def test_isolated_and_short_rows():
     assert nn[1].tolist() == [-1] * 5
     nn, _ = orc.knearest_many(x[:1], [10, 10, 10], 3)
     assert nn.tolist() == [[-1, -1, -1]]
+
+
+# ------------------------------------- KHeap ties at the k-th boundary (a7)
+
+TIES = json.loads((GOLD / "tie_answers.json").read_text())["cases"]
+PLANTS = {"replace_on_equal": 1, "last_argmax": 2, "canonical_index": 3,
+          "increasing_lists": 4, "finish_descending_index": 5}
+
+
+def _tie_rows_hold(cs, lib_path=None):
+    nn, d2 = orc.knearest_many(np.array(cs["xyz"], dtype=np.float64), cs["cell"],
+                               cs["k"], cell_hint=cs["hint"], lib_path=lib_path)
+    ok = True
+    for r, want in cs["rows"].items():
+        ok &= nn[int(r)].tolist() == want
+        ok &= d2[int(r)].tolist() == cs["d2"][r]
+    return ok
+
+
+@pytest.mark.parametrize("cs", TIES, ids=[c["name"] for c in TIES])
+def test_tie_answers(cs):
+    # hand-derived rows (tests/golden/make_golden.py): exact equality, every
+    # d2 is a sum of squares of multiples of 0.5
+    assert _tie_rows_hold(cs), cs["derivation"]
+
+
+@pytest.fixture(scope="module")
+def planted(tmp_path_factory):
+    """The oracle source built once per planted mistake (on the CPU)."""
+    import subprocess
+    src = Path(orc.__file__).resolve().parent / "lc_oracle.c"
+    d = tmp_path_factory.mktemp("plants")
+    out = {}
+    for name, code in PLANTS.items():
+        so = d / f"liblcoracle_{name}.so"
+        subprocess.run(["gcc", "-O2", "-std=c11", "-ffp-contract=off", "-fPIC", "-shared",
+                        "-pthread", f"-DLC_ORACLE_PLANT={code}", "-o", str(so), str(src),
+                        "-lm"], check=True)
+        out[name] = so
+    return out
+
+
+@pytest.mark.parametrize("plant", sorted(PLANTS))
+def test_tie_answers_reject_planted_mistakes(planted, plant):
+    # every misreading of the tie rules fails the fixtures that name it,
+    # and at least one fixture
+    failed = [cs["name"] for cs in TIES if not _tie_rows_hold(cs, planted[plant])]
+    named = [cs["name"] for cs in TIES if plant in cs["fails_if"]]
+    assert named, plant
+    assert set(named) <= set(failed), (plant, named, failed)
