@@ -132,6 +132,23 @@ int lc_gpu_sched_tile_count(lc_gpu_workspace *ws, size_t *n_tiles);
  * be valid. Waits for completion. */
 int lc_gpu_sched_tile_labels(lc_gpu_workspace *ws, float *out, size_t n);
 
+/* A search that allocates nothing, for callers that cannot: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation.
+ * No counterpart in the reference. Arguments as lc_gpu_knearest_many, with
+ * no workspace: the work goes onto `stream` (a hipStream_t; NULL is the
+ * null stream) and is not waited for. Every buffer of the search lives in
+ * the library's fixed device scratch of lc_gpu_noalloc_scratch_bytes()
+ * (64 MiB) per GPU, held until the process ends; a search that needs more
+ * fails with a message that says how much (roughly n_frames * n above
+ * 400,000 points). Calls on one GPU share the scratch: calls that overlap
+ * on that GPU race. Besides the frames' own inputs and outputs the call
+ * makes no device or host allocation on success. */
+int lc_gpu_knearest_noalloc(const double *xyz, size_t n, size_t n_frames,
+                            const struct lc_cell *simbox, size_t k, const int *mask,
+                            double cell_hint, int *out_nn, double *out_d2,
+                            void *stream);
+size_t lc_gpu_noalloc_scratch_bytes(void);
+
 /* Stage timings (milliseconds) of up to n_runs of the most recent timed
  * searches of this workspace, oldest first, 10 per run: fold, scan,
  * scatter, sort, walk, exact, total, schedule set-up, first walk pass,

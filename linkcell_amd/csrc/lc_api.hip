@@ -572,6 +572,66 @@ int lc_gpu_sched_tile_labels(lc_gpu_workspace* ws, float* out, size_t n) {
   });
 }
 
+int lc_gpu_knearest_noalloc(const double* xyz, size_t n, size_t n_frames,
+                            const struct lc_cell* simbox, size_t k, const int* mask,
+                            double cell_hint, int* out_nn, double* out_d2,
+                            void* stream) {
+  // validation in the order of the device entry points
+  // (src/gpu/knearest.cpp:711-762), with no frame vector
+  if (!device_present()) {
+    set_gpu_err(kNoDevice);
+    return 1;
+  }
+  if (n == 0 || n_frames == 0) {
+    set_gpu_err("no points");
+    return 1;
+  }
+  if (k == 0) {
+    set_gpu_err("k must be at least 1");
+    return 1;
+  }
+  if (n > SIZE_MAX / n_frames || n * n_frames > SIZE_MAX / k) {
+    set_gpu_err("n * k overflows");
+    return 1;
+  }
+  if (xyz == nullptr || simbox == nullptr || out_nn == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  if (n * n_frames >= static_cast<size_t>(INT_MAX) || k >= static_cast<size_t>(INT_MAX)) {
+    set_gpu_err("n * n_frames and k must be below 2^31 - 1");
+    return 1;
+  }
+  const lc_cell& cb = *simbox;
+  const double a[3] = {cb.ax, cb.ay, cb.az}, b[3] = {cb.bx, cb.by, cb.bz},
+               c[3] = {cb.cx, cb.cy, cb.cz}, o[3] = {cb.ox, cb.oy, cb.oz};
+  lcamd::FrameSpec s;
+  if (lcamd::cell_from_vectors(a, b, c, o, s.geom) != lcamd::kOk) {
+    set_gpu_err("bad cell");
+    return 1;
+  }
+  if (lcamd::plan_mesh(s.geom, hint_option(cell_hint), s.plan) != lcamd::kOk) {
+    set_gpu_err("too many cells");
+    return 1;
+  }
+  s.xyz = xyz;
+  s.mask = mask;
+  s.out_nn = out_nn;
+  s.out_d2 = out_d2;
+  s.n = static_cast<int>(n);
+  s.k = static_cast<int>(k);
+  try {
+    lcamd::search_noalloc(s, static_cast<int>(n_frames), static_cast<hipStream_t>(stream));
+  } catch (const std::exception& e) {
+    set_gpu_err(e.what());
+    return 1;
+  }
+  set_gpu_err(nullptr);
+  return 0;
+}
+
+size_t lc_gpu_noalloc_scratch_bytes(void) { return lcamd::kNoAllocScratchBytes; }
+
 int lc_gpu_timing_history(lc_gpu_workspace* ws, double* ms, int n_runs, int* got) {
   if (ws == nullptr || ms == nullptr || got == nullptr) {
     set_gpu_err("null pointer");

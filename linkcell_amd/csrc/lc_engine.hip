@@ -41,6 +41,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <mutex>
 #include <type_traits>
 
 #pragma clang fp contract(off)
@@ -2815,6 +2816,260 @@ std::vector<Timings> Engine::timing_history(int n) {
   return out;
 }
 
+namespace {
+
+// Sizes of one call's device buffers.
+struct Sizes {
+  int F = 0;
+  long long ntot = 0, ctot = 0;
+  long long tiles = 0;      // scan tiles
+  long long tcap = 0;       // bound on 16-point source tiles
+  long long max_items = 0;  // bound on 64-source work items
+  long long exact_threads = 0;
+  int kmax_exact = 1;
+};
+
+Sizes sizes_of(int F, long long ntot, long long ctot, int kmax_exact,
+               long long heap_budget) {
+  Sizes z;
+  z.F = F;
+  z.ntot = ntot;
+  z.ctot = ctot;
+  z.kmax_exact = kmax_exact;
+  z.tiles = ctot / kScanTile + 1;
+  // 16-point tiles: at most one partial tile per non-empty cell
+  z.tcap = ntot / kTile + std::min(ntot, ctot) + 1;
+  z.max_items = std::min<long long>(ntot, ctot + ntot / kWave + 1) + 1;
+  // exact-walk scratch: one heap of kmax slots per resident lane
+  constexpr long long kExactBlocks = 1024;
+  z.exact_threads = kExactBlocks * 64;
+  while (z.exact_threads > 64 && z.exact_threads * kmax_exact * 12 > heap_budget) {
+    z.exact_threads /= 2;
+  }
+  return z;
+}
+
+// Device buffers of one call, owned by an engine or carved from a fixed
+// scratch region. Order and sizes: buffer_bytes.
+struct Bufs {
+  FrameDesc* frames;
+  long long* fpt;
+  long long* fcb;
+  int* count;
+  unsigned long long* status;
+  int* counters;
+  int* off;
+  int* tile_off;
+  int* item_off;
+  int* items;
+  int4* cellslot;
+  long long* tmp_key;
+  Cand* cand;
+  int* scell;
+  int* byidx;
+  Box* tbox;
+  int* tile_cell;
+  Box* cbox;
+  int* defer;
+  int* flags;
+  double* heap_d2;
+  int* heap_idx;
+};
+constexpr int kNumBufs = sizeof(Bufs) / sizeof(void*);
+
+void buffer_bytes(const Sizes& z, size_t (&b)[kNumBufs]) {
+  const size_t F = static_cast<size_t>(z.F), N = static_cast<size_t>(z.ntot),
+               C = static_cast<size_t>(z.ctot);
+  const size_t hs = static_cast<size_t>(z.exact_threads) * static_cast<size_t>(z.kmax_exact);
+  const size_t v[kNumBufs] = {
+      sizeof(FrameDesc) * F,                         // frames
+      sizeof(long long) * (F + 1),                   // fpt
+      sizeof(long long) * (F + 1),                   // fcb
+      sizeof(int) * C,                               // count
+      sizeof(unsigned long long) * 2 * static_cast<size_t>(z.tiles),  // status
+      sizeof(int) * 4,                               // counters
+      sizeof(int) * (C + 1),                         // off
+      sizeof(int) * (C + 1),                         // tile_off
+      sizeof(int) * (C + 1),                         // item_off
+      sizeof(int) * static_cast<size_t>(z.max_items),  // items
+      sizeof(int4) * N,                              // cellslot
+      sizeof(long long) * N,                         // tmp_key
+      sizeof(Cand) * (N + kPad),                     // cand
+      sizeof(int) * N,                               // scell
+      sizeof(int) * N,                               // byidx
+      sizeof(Box) * static_cast<size_t>(z.tcap),     // tbox
+      sizeof(int) * static_cast<size_t>(z.tcap),     // tile_cell
+      sizeof(Box) * C,                               // cbox
+      sizeof(int) * N,                               // defer
+      sizeof(int) * F,                               // flags
+      sizeof(double) * hs,                           // heap_d2
+      sizeof(int) * hs,                              // heap_idx
+  };
+  for (int i = 0; i < kNumBufs; ++i) b[i] = v[i];
+}
+
+// Lays the buffers out from base (256-byte aligned each); returns the bytes
+// used. base may be 0 to measure.
+size_t carve(const Sizes& z, uintptr_t base, Bufs& out) {
+  size_t b[kNumBufs];
+  buffer_bytes(z, b);
+  void** slot = reinterpret_cast<void**>(&out);
+  size_t used = 0;
+  for (int i = 0; i < kNumBufs; ++i) {
+    slot[i] = reinterpret_cast<void*>(base + used);
+    used += (std::max<size_t>(b[i], 1) + 255) & ~static_cast<size_t>(255);
+  }
+  return used;
+}
+
+// Clears, fold/count, scans, scatter, cell sort and boxes. ev: the call's
+// timing events (null = untimed); 0..4 are recorded here.
+void launch_prep(const Sizes& z, const Bufs& B, hipStream_t st, hipEvent_t* ev) {
+  const long long ctot = z.ctot, ntot = z.ntot, tiles = z.tiles;
+  const int F = z.F;
+  if (ev) check(hipEventRecord(ev[0], st), "event record");
+  // clears (fill kernels, not memset nodes, so the sequence can be
+  // captured into a graph)
+  k_zero_u32<<<blocks_for(ctot, 256), 256, 0, st>>>(reinterpret_cast<unsigned*>(B.count),
+                                                    ctot);
+  k_zero_u32<<<blocks_for(tiles * 4, 256), 256, 0, st>>>(
+      reinterpret_cast<unsigned*>(B.status), tiles * 4);
+  k_zero_u32<<<1, 64, 0, st>>>(reinterpret_cast<unsigned*>(B.counters), 4);
+  k_zero_u32<<<blocks_for(F, 256), 256, 0, st>>>(reinterpret_cast<unsigned*>(B.flags), F);
+
+  k_fold_count<<<blocks_for(ntot, 256), 256, 0, st>>>(B.frames, B.fpt, F, ntot, B.count,
+                                                      B.cellslot, B.flags);
+  check(hipGetLastError(), "fold_count launch");
+  if (ev) check(hipEventRecord(ev[1], st), "event record");
+
+  k_scan<<<static_cast<unsigned>(tiles), kScanThreads, 0, st>>>(
+      B.count, ctot, kTileShift, B.off, B.tile_off, B.status, B.counters + 0);
+  k_scan<<<static_cast<unsigned>(tiles), kScanThreads, 0, st>>>(
+      B.count, ctot, kChunkShift, nullptr, B.item_off, B.status + tiles, B.counters + 2);
+  check(hipGetLastError(), "scan launch");
+  if (ev) check(hipEventRecord(ev[2], st), "event record");
+
+  k_emit_items<<<blocks_for(ctot, 256), 256, 0, st>>>(B.count, B.item_off, B.tile_off, ctot,
+                                                      B.items, B.tile_cell);
+  k_scatter<<<blocks_for(ntot, 256), 256, 0, st>>>(B.cellslot, B.fpt, F, ntot, B.off,
+                                                   B.tmp_key);
+  check(hipGetLastError(), "scatter launch");
+  if (ev) check(hipEventRecord(ev[3], st), "event record");
+
+  const unsigned item_blocks = blocks_for(z.max_items * kWave, 256);
+  k_cell_sort<<<item_blocks, 256, 0, st>>>(B.items, B.item_off, ctot, B.off, B.tmp_key,
+                                           B.frames, B.fcb, F, B.cand, B.scell, B.byidx);
+  k_tile_box<<<item_blocks, 256, 0, st>>>(B.items, B.item_off, ctot, B.off, B.tile_off,
+                                          B.cand, B.tbox);
+  k_cell_box<<<blocks_for(ctot, 256), 256, 0, st>>>(B.off, B.tile_off, ctot, B.tbox, B.cbox);
+  check(hipGetLastError(), "cell_sort launch");
+  if (ev) check(hipEventRecord(ev[4], st), "event record");
+}
+
+// Walk arguments over the call's buffers, plain schedule, no counters.
+WalkArgs walk_args(const Sizes& z, const Bufs& B, int dense_min) {
+  WalkArgs wa;
+  std::memset(&wa, 0, sizeof wa);
+  wa.fr = B.frames;
+  wa.fcb = B.fcb;
+  wa.F = z.F;
+  wa.fflags = B.flags;
+  wa.cand = B.cand;
+  wa.scell = B.scell;
+  wa.off = B.off;
+  wa.tile_off = B.tile_off;
+  wa.items = B.items;
+  wa.item_off = B.item_off;
+  wa.ncells = z.ctot;
+  wa.tbox = B.tbox;
+  wa.cbox = B.cbox;
+  wa.dense_min = dense_min;
+  wa.tile_cell = B.tile_cell;
+  wa.defer_list = B.defer;
+  wa.defer_count = B.counters + 1;
+  // capped row walk: rows whose cap may have hidden the reference's stop
+  // are walked again, uncapped, by their own wave; counted here
+  wa.retry_count = B.counters + 3;
+  return wa;
+}
+
+// The fast walks of every register bucket in use, then the exact-order
+// walk of the deferred and unbucketed sources. ev: events 8, 9, 5, 6, 7.
+void launch_walk(const Sizes& z, const Bufs& B, const WalkArgs& wa,
+                 const bool (&any_fast)[33], bool any_exact_frame, hipStream_t st,
+                 hipEvent_t* ev) {
+  const bool row_mode = wa.dense_min == kRowWalk;
+  const unsigned sparse_blocks = blocks_for(z.ntot, 256);
+  // row walk: one wave per 4 source tiles, frames on wave boundaries
+  const unsigned row_blocks = blocks_for(z.tcap / 4 + z.F + 1, 4);
+  if (ev) check(hipEventRecord(ev[8], st), "event record");
+#define LC_WALK(KC)                                                             \
+  if (any_fast[KC] && row_mode) {                                               \
+    if (wa.stats != nullptr) {                                                  \
+      k_walk_rows<KC, true><<<row_blocks, 256, 0, st>>>(wa);                    \
+    } else {                                                                    \
+      k_walk_rows<KC, false><<<row_blocks, 256, 0, st>>>(wa);                   \
+    }                                                                           \
+    check(hipGetLastError(), "walk launch");                                    \
+  } else if (any_fast[KC]) {                                                    \
+    constexpr int W = WalkShape<KC>::kWaves;                                    \
+    k_walk_dense<KC><<<static_cast<unsigned>((z.max_items + W - 1) / W),        \
+                       W * kWave, 0, st>>>(wa);                                 \
+    k_walk_sparse<KC><<<blocks_for(z.ntot, W * kWave), W * kWave, 0, st>>>(wa); \
+    check(hipGetLastError(), "walk launch");                                    \
+  }
+  LC_WALK(4)
+  LC_WALK(8)
+  LC_WALK(16)
+  LC_WALK(32)
+#undef LC_WALK
+  if (ev) check(hipEventRecord(ev[9], st), "event record");
+  // frames outside every register bucket go straight to the exact walk
+  if (any_exact_frame) {
+    k_route_exact<<<sparse_blocks, 256, 0, st>>>(B.frames, B.fcb, z.F, B.scell, B.off,
+                                                 z.ctot, B.defer, B.counters + 1);
+    check(hipGetLastError(), "route launch");
+  }
+  if (ev) check(hipEventRecord(ev[5], st), "event record");
+  k_walk_exact_wave<<<2048, 64, 0, st>>>(B.frames, B.fcb, z.F, B.cand, B.scell, B.off,
+                                          B.byidx, B.defer, B.counters + 1);
+  if (z.kmax_exact > kExactWaveMaxK) {
+    k_walk_exact<<<static_cast<unsigned>(z.exact_threads / 64), 64, 0, st>>>(
+        B.frames, B.fcb, z.F, B.cand, B.scell, B.off, B.byidx, B.defer, B.counters + 1,
+        B.heap_d2, B.heap_idx);
+  }
+  check(hipGetLastError(), "exact walk launch");
+  if (ev) {
+    check(hipEventRecord(ev[6], st), "event record");
+    check(hipEventRecord(ev[7], st), "event record");
+  }
+}
+
+// Frame descriptor of one frame spec (bases and bucket left to the caller).
+void fill_desc(const FrameSpec& s, FrameDesc& d) {
+  std::memset(&d, 0, sizeof d);
+  d.xyz = s.xyz;
+  d.mask = s.mask;
+  d.out_nn = s.out_nn;
+  d.out_d2 = s.out_d2;
+  d.out_reach = s.out_reach;
+  std::memcpy(d.h, s.geom.h, sizeof d.h);
+  std::memcpy(d.hinv, s.geom.hinv, sizeof d.hinv);
+  std::memcpy(d.origin, s.geom.origin, sizeof d.origin);
+  std::memcpy(d.widths, s.geom.widths, sizeof d.widths);
+  d.ortho = s.geom.ortho;
+  d.cell_min = s.plan.cell_min;
+  d.nx = s.plan.nx;
+  d.ny = s.plan.ny;
+  d.nz = s.plan.nz;
+  d.max_reach = s.plan.max_reach;
+  d.r_safe = safe_reach(s.plan);
+  d.n = s.n;
+  d.k = s.k;
+}
+
+}  // namespace
+
 void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   const int F = static_cast<int>(frames.size());
   if (F == 0) return;
@@ -2825,32 +3080,16 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   long long ntot = 0, ctot = 0;
   int kmax_exact = 1;
   bool any_fast[33] = {};
+  bool any_exact_frame = false;
   for (int f = 0; f < F; ++f) {
     const FrameSpec& s = frames[static_cast<size_t>(f)];
     FrameDesc& d = desc[static_cast<size_t>(f)];
-    std::memset(&d, 0, sizeof d);
-    d.xyz = s.xyz;
-    d.mask = s.mask;
-    d.out_nn = s.out_nn;
-    d.out_d2 = s.out_d2;
-    d.out_reach = s.out_reach;
-    std::memcpy(d.h, s.geom.h, sizeof d.h);
-    std::memcpy(d.hinv, s.geom.hinv, sizeof d.hinv);
-    std::memcpy(d.origin, s.geom.origin, sizeof d.origin);
-    std::memcpy(d.widths, s.geom.widths, sizeof d.widths);
-    d.ortho = s.geom.ortho;
-    d.cell_min = s.plan.cell_min;
-    d.nx = s.plan.nx;
-    d.ny = s.plan.ny;
-    d.nz = s.plan.nz;
-    d.max_reach = s.plan.max_reach;
-    d.r_safe = safe_reach(s.plan);
-    d.n = s.n;
-    d.k = s.k;
+    fill_desc(s, d);
     d.pt_base = ntot;
     d.cell_base = ctot;
     d.kc = mode_ == WalkMode::kExactOnly ? 0 : fast_bucket(s.k);
     if (d.kc) any_fast[d.kc] = true;
+    any_exact_frame |= d.kc == 0;
     kmax_exact = std::max(kmax_exact, s.k);
     fpt[static_cast<size_t>(f)] = ntot;
     fcb[static_cast<size_t>(f)] = ctot;
@@ -2865,43 +3104,40 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   last_F_ = F;
   last_ntot_ = ntot;
   last_ctot_ = ctot;
-  const long long tiles = ctot / kScanTile + 1;
-  // 16-point tiles: at most one partial tile per non-empty cell
-  const long long tcap = ntot / kTile + std::min(ntot, ctot) + 1;
-  const long long max_items = std::min<long long>(ntot, ctot + ntot / kWave + 1) + 1;
-
-  grow(frames_, sizeof(FrameDesc) * static_cast<size_t>(F));
-  grow(fpt_, sizeof(long long) * (static_cast<size_t>(F) + 1));
-  grow(fcb_, sizeof(long long) * (static_cast<size_t>(F) + 1));
-  grow(count_, sizeof(int) * static_cast<size_t>(ctot));
-  grow(status_, sizeof(unsigned long long) * 2 * static_cast<size_t>(tiles));
-  grow(counters_, sizeof(int) * 4);
-  grow(off_, sizeof(int) * (static_cast<size_t>(ctot) + 1));
-  grow(tile_off_, sizeof(int) * (static_cast<size_t>(ctot) + 1));
-  grow(item_off_, sizeof(int) * (static_cast<size_t>(ctot) + 1));
-  grow(items_, sizeof(int) * static_cast<size_t>(max_items));
-  grow(cellslot_, sizeof(int4) * static_cast<size_t>(ntot));
-  grow(tmp_key_, sizeof(long long) * static_cast<size_t>(ntot));
-  grow(cand_, sizeof(Cand) * static_cast<size_t>(ntot + kPad));
-  grow(scell_, sizeof(int) * static_cast<size_t>(ntot));
-  grow(byidx_, sizeof(int) * static_cast<size_t>(ntot));
-  grow(tbox_, sizeof(Box) * static_cast<size_t>(tcap));
-  grow(tile_cell_, sizeof(int) * static_cast<size_t>(tcap));
-  grow(cbox_, sizeof(Box) * static_cast<size_t>(ctot));
-  grow(defer_, sizeof(int) * static_cast<size_t>(ntot));
-  grow(flags_, sizeof(int) * static_cast<size_t>(F));
-  // exact-walk scratch: one heap of kmax slots per resident lane
-  constexpr long long kExactBlocks = 1024;
-  long long exact_threads = kExactBlocks * 64;
-  const long long heap_budget = 512ll << 20;
-  while (exact_threads > 64 &&
-         exact_threads * kmax_exact * 12 > heap_budget) {
-    exact_threads /= 2;
+  const Sizes z = sizes_of(F, ntot, ctot, kmax_exact, 512ll << 20);
+  const long long tcap = z.tcap;
+  {
+    size_t bytes[kNumBufs];
+    buffer_bytes(z, bytes);
+    Buf* own[kNumBufs] = {&frames_,  &fpt_,   &fcb_,      &count_,    &status_, &counters_,
+                          &off_,     &tile_off_, &item_off_, &items_,   &cellslot_,
+                          &tmp_key_, &cand_,  &scell_,    &byidx_,    &tbox_,   &tile_cell_,
+                          &cbox_,    &defer_, &flags_,    &heap_d2_,  &heap_idx_};
+    for (int i = 0; i < kNumBufs; ++i) grow(*own[i], bytes[i]);
   }
-  grow(heap_d2_, sizeof(double) * static_cast<size_t>(exact_threads) *
-                     static_cast<size_t>(kmax_exact));
-  grow(heap_idx_, sizeof(int) * static_cast<size_t>(exact_threads) *
-                      static_cast<size_t>(kmax_exact));
+  Bufs B;
+  B.frames = as<FrameDesc>(frames_.p);
+  B.fpt = as<long long>(fpt_.p);
+  B.fcb = as<long long>(fcb_.p);
+  B.count = as<int>(count_.p);
+  B.status = as<unsigned long long>(status_.p);
+  B.counters = as<int>(counters_.p);
+  B.off = as<int>(off_.p);
+  B.tile_off = as<int>(tile_off_.p);
+  B.item_off = as<int>(item_off_.p);
+  B.items = as<int>(items_.p);
+  B.cellslot = as<int4>(cellslot_.p);
+  B.tmp_key = as<long long>(tmp_key_.p);
+  B.cand = as<Cand>(cand_.p);
+  B.scell = as<int>(scell_.p);
+  B.byidx = as<int>(byidx_.p);
+  B.tbox = as<Box>(tbox_.p);
+  B.tile_cell = as<int>(tile_cell_.p);
+  B.cbox = as<Box>(cbox_.p);
+  B.defer = as<int>(defer_.p);
+  B.flags = as<int>(flags_.p);
+  B.heap_d2 = as<double>(heap_d2_.p);
+  B.heap_idx = as<int>(heap_idx_.p);
 
   // descriptors go through pinned staging; wait until the previous call's
   // copy has consumed it
@@ -2937,19 +3173,15 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   std::memcpy(sp, desc.data(), dbytes);
   std::memcpy(sp + dbytes, fpt.data(), bbytes);
   std::memcpy(sp + dbytes + bbytes, fcb.data(), bbytes);
-  check(hipMemcpyAsync(frames_.p, sp, dbytes, hipMemcpyHostToDevice, st),
-        "HtoD frames");
-  check(hipMemcpyAsync(fpt_.p, sp + dbytes, bbytes, hipMemcpyHostToDevice, st),
-        "HtoD fpt");
-  check(hipMemcpyAsync(fcb_.p, sp + dbytes + bbytes, bbytes,
-                       hipMemcpyHostToDevice, st),
+  check(hipMemcpyAsync(B.frames, sp, dbytes, hipMemcpyHostToDevice, st), "HtoD frames");
+  check(hipMemcpyAsync(B.fpt, sp + dbytes, bbytes, hipMemcpyHostToDevice, st), "HtoD fpt");
+  check(hipMemcpyAsync(B.fcb, sp + dbytes + bbytes, bbytes, hipMemcpyHostToDevice, st),
         "HtoD fcb");
   if (!capturing_) {
     check(hipEventRecord(staging_ev_, st), "event record");
     staging_pending_ = true;
   }
   const bool timing = timing_ && !capturing_;
-
   if (timing) {
     // a slot comes round again kTimingRing timed runs later (recording an
     // event again while its last record is pending is allowed)
@@ -2957,84 +3189,15 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
     for (int e = 0; e < 10; ++e) {
       if (ev_[e] == nullptr) check(hipEventCreate(&ev_[e]), "hipEventCreate");
     }
-    check(hipEventRecord(ev_[0], st), "event record");
   }
+  hipEvent_t* ev = timing ? ev_ : nullptr;
 
-  // clears (fill kernels, not memset nodes, so the sequence can be
-  // captured into a graph)
-  k_zero_u32<<<blocks_for(ctot, 256), 256, 0, st>>>(as<unsigned>(count_.p),
-                                                    ctot);
-  k_zero_u32<<<blocks_for(tiles * 4, 256), 256, 0, st>>>(
-      as<unsigned>(status_.p), tiles * 4);
-  k_zero_u32<<<1, 64, 0, st>>>(as<unsigned>(counters_.p), 4);
-  k_zero_u32<<<blocks_for(F, 256), 256, 0, st>>>(as<unsigned>(flags_.p), F);
+  launch_prep(z, B, st, ev);
 
-  const FrameDesc* dfr = as<FrameDesc>(frames_.p);
-  const long long* dfpt = as<long long>(fpt_.p);
-  const long long* dfcb = as<long long>(fcb_.p);
-  int* counters = as<int>(counters_.p);
-  unsigned long long* status = as<unsigned long long>(status_.p);
-
-  k_fold_count<<<blocks_for(ntot, 256), 256, 0, st>>>(
-      dfr, dfpt, F, ntot, as<int>(count_.p), as<int4>(cellslot_.p),
-      as<int>(flags_.p));
-  check(hipGetLastError(), "fold_count launch");
-  if (timing) check(hipEventRecord(ev_[1], st), "event record");
-
-  k_scan<<<static_cast<unsigned>(tiles), kScanThreads, 0, st>>>(
-      as<int>(count_.p), ctot, kTileShift, as<int>(off_.p),
-      as<int>(tile_off_.p), status, counters + 0);
-  k_scan<<<static_cast<unsigned>(tiles), kScanThreads, 0, st>>>(
-      as<int>(count_.p), ctot, kChunkShift, nullptr, as<int>(item_off_.p),
-      status + tiles, counters + 2);
-  check(hipGetLastError(), "scan launch");
-  if (timing) check(hipEventRecord(ev_[2], st), "event record");
-
-  k_emit_items<<<blocks_for(ctot, 256), 256, 0, st>>>(
-      as<int>(count_.p), as<int>(item_off_.p), as<int>(tile_off_.p), ctot,
-      as<int>(items_.p), as<int>(tile_cell_.p));
-  k_scatter<<<blocks_for(ntot, 256), 256, 0, st>>>(
-      as<int4>(cellslot_.p), dfpt, F, ntot, as<int>(off_.p),
-      as<long long>(tmp_key_.p));
-  check(hipGetLastError(), "scatter launch");
-  if (timing) check(hipEventRecord(ev_[3], st), "event record");
-
-  const unsigned item_blocks = blocks_for(max_items * kWave, 256);
-  k_cell_sort<<<item_blocks, 256, 0, st>>>(
-      as<int>(items_.p), as<int>(item_off_.p), ctot, as<int>(off_.p),
-      as<long long>(tmp_key_.p), dfr, dfcb, F, as<Cand>(cand_.p),
-      as<int>(scell_.p), as<int>(byidx_.p));
-  k_tile_box<<<item_blocks, 256, 0, st>>>(
-      as<int>(items_.p), as<int>(item_off_.p), ctot, as<int>(off_.p),
-      as<int>(tile_off_.p), as<Cand>(cand_.p), as<Box>(tbox_.p));
-  k_cell_box<<<blocks_for(ctot, 256), 256, 0, st>>>(
-      as<int>(off_.p), as<int>(tile_off_.p), ctot, as<Box>(tbox_.p),
-      as<Box>(cbox_.p));
-  check(hipGetLastError(), "cell_sort launch");
-  if (timing) check(hipEventRecord(ev_[4], st), "event record");
-
-  WalkArgs wa;
-  wa.fr = dfr;
-  wa.fcb = dfcb;
-  wa.F = F;
-  wa.fflags = as<int>(flags_.p);
-  wa.cand = as<Cand>(cand_.p);
-  wa.scell = as<int>(scell_.p);
-  wa.off = as<int>(off_.p);
-  wa.tile_off = as<int>(tile_off_.p);
-  wa.items = as<int>(items_.p);
-  wa.item_off = as<int>(item_off_.p);
-  wa.ncells = ctot;
-  wa.tbox = as<Box>(tbox_.p);
-  wa.cbox = as<Box>(cbox_.p);
   // walk split: kRowWalk = rows everywhere; positive = a wave per dense
   // cell, one lane per source elsewhere
-  wa.dense_min = dense_min_for(frames);
+  WalkArgs wa = walk_args(z, B, dense_min_for(frames));
   const bool row_mode = wa.dense_min == kRowWalk;
-  wa.tile_cell = as<int>(tile_cell_.p);
-  wa.defer_list = as<int>(defer_.p);
-  wa.defer_count = counters + 1;
-  wa.stats = nullptr;
   if (stats_on_ && !capturing_) {
     const long long nstat = kStatCount + 2 * (ntot / kWave + 1);
     grow(stats_, sizeof(unsigned long long) * static_cast<size_t>(nstat));
@@ -3043,12 +3206,6 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
     stats_n_ = nstat;
     wa.stats = as<unsigned long long>(stats_.p);
   }
-  wa.sched = nullptr;
-  wa.sched_count = nullptr;
-  wa.pred = nullptr;
-  wa.pred_tile = nullptr;
-  wa.pred_tile_n = 0;
-  wa.retry_count = nullptr;
   const float* pred = nullptr;
   if (sched_mode_ == 1 && pred_ != nullptr && pred_->trained() && row_mode) {
     // learned caps of every source tile, from the counts and tile boxes
@@ -3071,9 +3228,6 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
     }
   }
   wa.pred = pred;
-  // capped row walk: rows whose cap may have hidden the reference's stop
-  // are walked again, uncapped, by their own wave; counted here
-  wa.retry_count = counters + 3;
   // depth order of the lane-per-source walk from the per-cell prediction
   if (pred != nullptr && !row_mode) {
     grow(sched_, sizeof(int) * static_cast<size_t>(ntot));
@@ -3081,70 +3235,114 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
     int* cnt = as<int>(sched_cnt_.p);
     k_zero_u32<<<1, 64, 0, st>>>(as<unsigned>(sched_cnt_.p), 2 * kSchedBuckets);
     k_sched<false><<<blocks_for(ntot, 256), 256, 0, st>>>(
-        dfr, dfcb, F, as<int>(scell_.p), as<int>(off_.p), ctot, pred,
-        wa.dense_min, cnt, cnt + kSchedBuckets, as<int>(sched_.p));
+        B.frames, B.fcb, F, B.scell, B.off, ctot, pred, wa.dense_min, cnt,
+        cnt + kSchedBuckets, as<int>(sched_.p));
     k_sched<true><<<blocks_for(ntot, 256), 256, 0, st>>>(
-        dfr, dfcb, F, as<int>(scell_.p), as<int>(off_.p), ctot, pred,
-        wa.dense_min, cnt, cnt + kSchedBuckets, as<int>(sched_.p));
+        B.frames, B.fcb, F, B.scell, B.off, ctot, pred, wa.dense_min, cnt,
+        cnt + kSchedBuckets, as<int>(sched_.p));
     check(hipGetLastError(), "schedule launch");
     wa.sched = as<int>(sched_.p);
     wa.sched_count = cnt;
   }
-  const unsigned sparse_blocks = blocks_for(ntot, 256);
-  // row walk: one wave per 4 source tiles, frames on wave boundaries
-  const unsigned row_blocks = blocks_for(tcap / 4 + F + 1, 4);
-  if (timing) check(hipEventRecord(ev_[8], st), "event record");
-#define LC_WALK(KC)                                                           \
-  if (any_fast[KC] && row_mode) {                                             \
-    if (wa.stats != nullptr) {                                                \
-      k_walk_rows<KC, true><<<row_blocks, 256, 0, st>>>(wa);                  \
-    } else {                                                                  \
-      k_walk_rows<KC, false><<<row_blocks, 256, 0, st>>>(wa);                 \
-    }                                                                         \
-    check(hipGetLastError(), "walk launch");                                  \
-  } else if (any_fast[KC]) {                                                  \
-    constexpr int W = WalkShape<KC>::kWaves;                                  \
-    k_walk_dense<KC><<<static_cast<unsigned>((max_items + W - 1) / W),        \
-                       W * kWave, 0, st>>>(wa);                               \
-    k_walk_sparse<KC><<<blocks_for(ntot, W * kWave), W * kWave, 0, st>>>(wa); \
-    check(hipGetLastError(), "walk launch");                                  \
-  }
-  LC_WALK(4)
-  LC_WALK(8)
-  LC_WALK(16)
-  LC_WALK(32)
-#undef LC_WALK
-  if (timing) check(hipEventRecord(ev_[9], st), "event record");
-  // frames outside every register bucket go straight to the exact walk
-  bool any_exact_frame = false;
-  for (int f = 0; f < F; ++f) {
-    any_exact_frame |= desc[static_cast<size_t>(f)].kc == 0;
-  }
-  if (any_exact_frame) {
-    k_route_exact<<<sparse_blocks, 256, 0, st>>>(
-        dfr, dfcb, F, as<int>(scell_.p), as<int>(off_.p), ctot,
-        as<int>(defer_.p), counters + 1);
-    check(hipGetLastError(), "route launch");
-  }
-  if (timing) check(hipEventRecord(ev_[5], st), "event record");
-  k_walk_exact_wave<<<2048, 64, 0, st>>>(
-      dfr, dfcb, F, as<Cand>(cand_.p), as<int>(scell_.p), as<int>(off_.p),
-      as<int>(byidx_.p), as<int>(defer_.p), counters + 1);
-  if (kmax_exact > kExactWaveMaxK) {
-    k_walk_exact<<<static_cast<unsigned>(exact_threads / 64), 64, 0, st>>>(
-        dfr, dfcb, F, as<Cand>(cand_.p), as<int>(scell_.p), as<int>(off_.p),
-        as<int>(byidx_.p),
-        as<int>(defer_.p), counters + 1, as<double>(heap_d2_.p),
-        as<int>(heap_idx_.p));
-  }
-  check(hipGetLastError(), "exact walk launch");
+  launch_walk(z, B, wa, any_fast, any_exact_frame, st, ev);
   if (timing) {
-    check(hipEventRecord(ev_[6], st), "event record");
-    check(hipEventRecord(ev_[7], st), "event record");
     ring_pos_ += 1;
     timings_pending_ = true;
   }
   if (wait && !capturing_) sync();
+}
+
+// ------------------------------------------------ fixed-scratch search
+//
+// A search that allocates nothing: every buffer of the pipeline is carved
+// from one file-scope device array, and the frame descriptors are written
+// by a kernel from its by-value arguments, so a call makes no hipMalloc,
+// hipFree, host-pinned allocation, event or stream. Each GPU holds its own
+// copy of the array until the process ends; the kernels receive its
+// per-device address (hipGetSymbolAddress), never the host-side name. All
+// calls on one GPU share it: calls that overlap on that GPU race.
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device.
+void* scratch_for_current_device() {
+  int dev = 0;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress");
+    g_scratch_ptr[dev] = ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+
+// Descriptors and bases of F frames sharing one cell: frame f's points,
+// outputs and bases follow frame 0's at strides n (points) and ncell.
+__global__ void k_noalloc_desc(FrameDesc base, int F, long long n, long long ncell,
+                               long long k, FrameDesc* __restrict__ out,
+                               long long* __restrict__ fpt, long long* __restrict__ fcb) {
+  const long long f = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  if (f > F) return;
+  fpt[f] = f * n;
+  fcb[f] = f * ncell;
+  if (f == F) return;
+  FrameDesc d = base;
+  d.xyz = base.xyz + f * n * 3;
+  d.out_nn = base.out_nn + f * n * k;
+  d.out_d2 = base.out_d2 != nullptr ? base.out_d2 + f * n * k : nullptr;
+  d.pt_base = f * n;
+  d.cell_base = f * ncell;
+  out[f] = d;
+}
+
+Sizes noalloc_sizes(int F, long long n, long long ncell, int k) {
+  // the exact walk's heaps (k above 128) take at most 8 MiB of the scratch
+  return sizes_of(F, static_cast<long long>(F) * n, static_cast<long long>(F) * ncell, k,
+                  8ll << 20);
+}
+
+}  // namespace
+
+size_t noalloc_bytes(int n_frames, const FrameSpec& frame0) {
+  const Sizes z = noalloc_sizes(n_frames, frame0.n, frame0.plan.ncell, frame0.k);
+  Bufs B;
+  return carve(z, 0, B);
+}
+
+void search_noalloc(const FrameSpec& s, int n_frames, hipStream_t st) {
+  const long long n = s.n, ncell = s.plan.ncell;
+  const Sizes z = noalloc_sizes(n_frames, n, ncell, s.k);
+  if (z.ntot >= INT_MAX / 2 || z.ctot >= INT_MAX / 2) {
+    throw std::runtime_error("batch exceeds 2^30 points or cells");
+  }
+  Bufs B;
+  const size_t need = carve(z, 0, B);
+  if (need > kNoAllocScratchBytes) {
+    throw std::runtime_error("the search needs " + std::to_string(need) +
+                             " bytes of scratch; the fixed scratch holds " +
+                             std::to_string(kNoAllocScratchBytes));
+  }
+  carve(z, reinterpret_cast<uintptr_t>(scratch_for_current_device()), B);
+  FrameDesc d;
+  fill_desc(s, d);
+  d.pt_base = 0;
+  d.cell_base = 0;
+  d.kc = fast_bucket(s.k);
+  bool any_fast[33] = {};
+  if (d.kc) any_fast[d.kc] = true;
+  k_noalloc_desc<<<blocks_for(n_frames + 1, 64), 64, 0, st>>>(d, n_frames, n, ncell, s.k,
+                                                              B.frames, B.fpt, B.fcb);
+  check(hipGetLastError(), "descriptor launch");
+  launch_prep(z, B, st, nullptr);
+  launch_walk(z, B, walk_args(z, B, kRowWalk), any_fast, d.kc == 0, st, nullptr);
 }
 
 namespace {

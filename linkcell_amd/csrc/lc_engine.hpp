@@ -221,4 +221,14 @@ class Engine {
 // Heap-capacity bucket of the register-resident fast walk for k.
 int fast_bucket(int k);
 
+// Search without allocating: n_frames frames sharing frame0's cell, k and
+// mask, frame f at strides n (points) and n * k (outputs), every buffer in
+// the library's fixed device scratch (kNoAllocScratchBytes per GPU), on
+// stream st, not waited for. Throws std::runtime_error when the search
+// needs more scratch (noalloc_bytes) or on a HIP failure. Calls that
+// overlap on one GPU share the scratch and race.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+size_t noalloc_bytes(int n_frames, const FrameSpec& frame0);
+void search_noalloc(const FrameSpec& frame0, int n_frames, hipStream_t st);
+
 }  // namespace lcamd
