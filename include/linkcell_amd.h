@@ -132,6 +132,22 @@ int lc_gpu_sched_tile_count(lc_gpu_workspace *ws, size_t *n_tiles);
  * be valid. Waits for completion. */
 int lc_gpu_sched_tile_labels(lc_gpu_workspace *ws, float *out, size_t n);
 
+/* Device-cell searches (lc_gpu_knearest_many_dcell) invert the cell and plan
+ * the mesh on the device, so nothing is read back and the search can be
+ * captured into a graph; their buffers are sized for this many cells per
+ * frame (default 262144, the reference's device limit), and a plan above it
+ * fails with "too many cells". The failure of a call made with wait = 0 is
+ * reported by the next lc_gpu_wait (through lc_gpu_last_error); its output
+ * rows are -1 / NaN. */
+int lc_gpu_set_dcell_capacity(lc_gpu_workspace *ws, size_t cells_per_frame);
+
+/* Diagnostics: the device's plan of a device cell as 36 doubles: H by
+ * columns (9), Hinv (9), origin (3), widths (3), ortho, cell_min, nx, ny,
+ * nz, cells, max_reach, status (0 planned, 1 bad cell, 2 too many cells),
+ * then 4 zeros. Waits. */
+int lc_gpu_dcell_plan(lc_gpu_workspace *ws, const double *cell, int cell_n,
+                      double cell_hint, double *out36);
+
 /* A search that allocates nothing, for callers that cannot: code under
  * HIP graph capture, tight serving loops, harnesses that forbid allocation.
  * No counterpart in the reference. Arguments as lc_gpu_knearest_many, with

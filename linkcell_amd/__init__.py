@@ -120,7 +120,12 @@ class Workspace:
         return int(self._lib.lc_gpu_queue(self._ws) or 0)
 
     def wait(self):
+        """Wait for the workspace stream; raises a failure reported there
+        (a device-cell plan of a call made without waiting)."""
         self._lib.lc_gpu_wait(self._ws)
+        msg = self._lib.lc_gpu_last_error()
+        if msg:
+            raise Error(msg.decode())
 
     def set_mode(self, mode: int):
         self._check(self._lib.lc_gpu_set_mode(self._ws, int(mode)))
@@ -321,6 +326,20 @@ class Workspace:
             1 if wait else 0)
         self._check(rc)
 
+    def set_dcell_capacity(self, cells_per_frame: int):
+        """Cells per frame the device-cell path sizes its buffers for."""
+        self._check(self._lib.lc_gpu_set_dcell_capacity(self._ws, int(cells_per_frame)))
+
+    def dcell_plan(self, cell_ptr: int, cell_n: int, cell_hint: float = 0.0) -> dict:
+        """The device's plan of a device cell (diagnostics)."""
+        out = (C.c_double * 36)()
+        self._check(self._lib.lc_gpu_dcell_plan(self._ws, C.c_void_p(cell_ptr), int(cell_n),
+                                                float(cell_hint), out))
+        v = list(out)
+        return dict(h=v[0:9], hinv=v[9:18], origin=v[18:21], widths=v[21:24],
+                    ortho=int(v[24]), cell_min=v[25], nx=int(v[26]), ny=int(v[27]),
+                    nz=int(v[28]), ncell=int(v[29]), max_reach=int(v[30]), status=int(v[31]))
+
     @staticmethod
     def _frames(frames):
         """Build an ``lc_gpu_frame`` array from dicts with keys xyz, n,
@@ -480,8 +499,10 @@ def _knearest_device(xyz, cell, k, mask, hint):
     queue.wait_event(ready)
     mptr = m_dev.data_ptr() if m_dev is not None else 0
     if dcell is not None:
+        # waits, as the reference binding does (python/src/gpu.rs:302-322),
+        # so a bad device cell raises here
         ws.knearest_dcell_ptr(xyz.data_ptr(), n, frames, dcell.data_ptr(), dcell.numel(),
-                              k, nn.data_ptr(), d2.data_ptr(), mptr, hint, wait=False)
+                              k, nn.data_ptr(), d2.data_ptr(), mptr, hint, wait=True)
     else:
         ws.knearest_ptr(xyz.data_ptr(), n, frames, c12, k, nn.data_ptr(), d2.data_ptr(),
                         mptr, hint, wait=False)

@@ -33,6 +33,7 @@ EXPORTS = (
     "lc_gpu_sched_forward", "lc_gpu_sched_gradient", "lc_gpu_sched_adam",
     "lc_gpu_last_timing_ex", "lc_gpu_sched_dump", "lc_gpu_timing_history",
     "lc_gpu_knearest_noalloc", "lc_gpu_noalloc_scratch_bytes",
+    "lc_gpu_set_dcell_capacity", "lc_gpu_dcell_plan",
 )
 
 
@@ -110,6 +111,8 @@ _PROTOS = {
     "lc_gpu_knearest_noalloc": (C.c_int, [_vp, _sz, _sz, C.POINTER(lc_cell), _sz, _vp,
                                           C.c_double, _vp, _vp, _vp]),
     "lc_gpu_noalloc_scratch_bytes": (_sz, []),
+    "lc_gpu_set_dcell_capacity": (C.c_int, [_vp, _sz]),
+    "lc_gpu_dcell_plan": (C.c_int, [_vp, _vp, C.c_int, C.c_double, _dp]),
     "lc_gpu_last_timing_ex": (C.c_int, [_vp, _dp, C.c_int]),
     "lc_gpu_sched_collect": (C.c_int, [_vp, C.POINTER(_sz)]),
     "lc_gpu_sched_samples": (C.c_int, [_vp, C.POINTER(_sz)]),

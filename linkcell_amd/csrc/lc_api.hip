@@ -83,7 +83,13 @@ Workspace& Workspace::operator=(Workspace&& o) noexcept {
 
 void* Workspace::queue() { return impl_->engine.stream(); }
 
-void Workspace::wait() { impl_->engine.sync(); }
+void Workspace::wait() {
+  try {
+    impl_->engine.sync();
+  } catch (const std::runtime_error& e) {
+    throw Error(e.what());
+  }
+}
 
 }  // namespace gpu
 }  // namespace linkcell
@@ -185,30 +191,29 @@ void Workspace::knearest_into_many_dcell(const double* xyz, std::size_t n,
   if (cell_n != 3 && cell_n != 9 && cell_n != 12) {
     throw Error("device cell must have 3, 9, or 12 doubles");
   }
-  // The cell is tiny; read it on the workspace stream (ordered after any
-  // producer the caller enqueued there) and plan on the host.
-  double h[12] = {0};
-  hipStream_t st = impl_->engine.stream();
-  if (hipMemcpyAsync(h, cell, sizeof(double) * static_cast<size_t>(cell_n),
-                     hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipStreamSynchronize(st) != hipSuccess) {
-    throw Error("DtoH cell");
+  if (n > SIZE_MAX / nFrames || n * nFrames > SIZE_MAX / k) throw Error("n * k overflows");
+  if (out_len != n * k * nFrames) {
+    throw Error("out buffer length must be nFrames * n * k");
   }
-  double a[3], b[3], c[3], o[3] = {0.0, 0.0, 0.0};
-  if (cell_n == 3) {
-    a[0] = h[0]; a[1] = 0.0; a[2] = 0.0;
-    b[0] = 0.0; b[1] = h[1]; b[2] = 0.0;
-    c[0] = 0.0; c[1] = 0.0; c[2] = h[2];
-  } else {
-    for (int t = 0; t < 3; ++t) {
-      a[t] = h[t];
-      b[t] = h[3 + t];
-      c[t] = h[6 + t];
-      if (cell_n == 12) o[t] = h[9 + t];
-    }
+  if (n * nFrames >= static_cast<size_t>(INT_MAX) || k >= static_cast<size_t>(INT_MAX)) {
+    throw Error("n * n_frames and k must be below 2^31 - 1");
   }
-  lcamd::search_device(impl_->engine, xyz, n, nFrames, a, b, c, o, k, out,
-                       out_len, mask, cell_hint, wait, nullptr, out_d2, nullptr);
+  // The cell stays on the device: it is inverted and the mesh planned
+  // there (no read-back, no host copy), so the search can be captured.
+  // A bad cell or too fine a mesh is reported after the wait.
+  lcamd::FrameSpec spec;
+  spec.xyz = xyz;
+  spec.mask = mask;
+  spec.out_nn = out;
+  spec.out_d2 = out_d2;
+  spec.n = static_cast<int>(n);
+  spec.k = static_cast<int>(k);
+  try {
+    impl_->engine.run_dcell(spec, static_cast<int>(nFrames), cell, cell_n,
+                            hint_option(cell_hint), wait);
+  } catch (const std::runtime_error& e) {
+    throw Error(e.what());
+  }
 }
 
 }  // namespace gpu
@@ -315,7 +320,10 @@ void lc_gpu_wait(lc_gpu_workspace* ws) {
   if (ws != nullptr) {
     try {
       ws->ws.wait();
+      set_gpu_err(nullptr);
     } catch (const std::exception& e) {
+      // a failure the stream reports (a device-cell plan of a call made
+      // with wait = 0) goes to the error slot
       set_gpu_err(e.what());
     }
   }
@@ -631,6 +639,39 @@ int lc_gpu_knearest_noalloc(const double* xyz, size_t n, size_t n_frames,
 }
 
 size_t lc_gpu_noalloc_scratch_bytes(void) { return lcamd::kNoAllocScratchBytes; }
+
+int lc_gpu_set_dcell_capacity(lc_gpu_workspace* ws, size_t cells_per_frame) {
+  if (ws == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  if (cells_per_frame == 0 || cells_per_frame > static_cast<size_t>(lcamd::kMaxCells)) {
+    set_gpu_err("cell capacity must be between 1 and 16777216");
+    return 1;
+  }
+  ws->ws.impl()->engine.set_dcell_capacity(static_cast<long long>(cells_per_frame));
+  set_gpu_err(nullptr);
+  return 0;
+}
+
+int lc_gpu_dcell_plan(lc_gpu_workspace* ws, const double* cell, int cell_n, double cell_hint,
+                      double* out36) {
+  if (ws == nullptr || cell == nullptr || out36 == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  if (cell_n != 3 && cell_n != 9 && cell_n != 12) {
+    set_gpu_err("device cell must have 3, 9, or 12 doubles");
+    return 1;
+  }
+  return guarded([&] {
+    try {
+      ws->ws.impl()->engine.dcell_plan(cell, cell_n, hint_option(cell_hint), out36);
+    } catch (const std::runtime_error& e) {
+      throw linkcell::Error(e.what());
+    }
+  });
+}
 
 int lc_gpu_timing_history(lc_gpu_workspace* ws, double* ms, int n_runs, int* got) {
   if (ws == nullptr || ms == nullptr || got == nullptr) {

@@ -259,7 +259,7 @@ __global__ __launch_bounds__(256) void k_fold_count(
   const int f = find_frame(fpt, F, p);
   const FrameDesc& d = fr[f];
   const int i = static_cast<int>(p - d.pt_base);
-  if (d.mask != nullptr && d.mask[i] == 0) {
+  if (d.skip != 0 || (d.mask != nullptr && d.mask[i] == 0)) {
     cellslot[p] = make_int4(-1, 0, 0, 0);
     const size_t row = static_cast<size_t>(i) * d.k;
     for (int t = 0; t < d.k; ++t) {
@@ -309,6 +309,74 @@ __global__ __launch_bounds__(256) void k_fold_count(
   }
   if (!runs) slot = atomicAdd(&count[gcell], 1);
   cellslot[p] = make_int4(static_cast<int>(gcell), slot, key, 0);
+}
+
+// Device-side plan of a device-resident cell: Cell::from_vectors and the
+// mesh plan (src/cell.rs:97-123, src/knearest.rs:377-397) with the host
+// plan's arithmetic (one thread; IEEE division and square root), then the
+// descriptors of every frame (the role of the reference's invert_cell,
+// src/gpu/knearest.cpp:74-190, without its read-back of the plan). status:
+// 0 planned, 1 bad cell, 2 too many cells (the reference's limits, or more
+// than cap_cells cells); a failed frame is marked skip over a 1x1x1 mesh.
+__global__ __launch_bounds__(64) void k_plan_dcell(
+    const double* __restrict__ cell, int cell_n, double hint, FrameDesc tmpl, int F,
+    long long n, long long k, long long cap_cells, FrameDesc* __restrict__ out,
+    long long* __restrict__ fpt, long long* __restrict__ fcb, int* __restrict__ status) {
+  __shared__ CellGeom sg;
+  __shared__ MeshPlan sp;
+  __shared__ int sst;
+  if (threadIdx.x == 0) {
+    double a[3], b[3], c[3], o[3] = {0.0, 0.0, 0.0};
+    if (cell_n == 3) {
+      a[0] = cell[0], a[1] = 0.0, a[2] = 0.0;
+      b[0] = 0.0, b[1] = cell[1], b[2] = 0.0;
+      c[0] = 0.0, c[1] = 0.0, c[2] = cell[2];
+    } else {
+      for (int t = 0; t < 3; ++t) {
+        a[t] = cell[t];
+        b[t] = cell[3 + t];
+        c[t] = cell[6 + t];
+        if (cell_n == 12) o[t] = cell[9 + t];
+      }
+    }
+    int st = cell_from_vectors(a, b, c, o, sg) == kOk ? 0 : 1;
+    if (st == 0 && plan_mesh(sg, hint, sp) != kOk) st = 2;
+    if (st == 0 && sp.ncell > cap_cells) st = 2;
+    sst = st;
+    *status = st;
+  }
+  __syncthreads();
+  const bool ok = sst == 0;
+  const long long nc = ok ? sp.ncell : 1;
+  for (int f = threadIdx.x; f <= F; f += blockDim.x) {
+    fpt[f] = f * n;
+    fcb[f] = f * nc;
+    if (f == F) continue;
+    FrameDesc d = tmpl;
+    d.xyz = tmpl.xyz + f * n * 3;
+    d.out_nn = tmpl.out_nn + f * n * k;
+    d.out_d2 = tmpl.out_d2 != nullptr ? tmpl.out_d2 + f * n * k : nullptr;
+    d.out_reach = tmpl.out_reach != nullptr ? tmpl.out_reach + f * n : nullptr;
+    for (int t = 0; t < 9; ++t) {
+      d.h[t] = ok ? sg.h[t] : (t % 4 == 0 ? 1.0 : 0.0);
+      d.hinv[t] = ok ? sg.hinv[t] : (t % 4 == 0 ? 1.0 : 0.0);
+    }
+    for (int t = 0; t < 3; ++t) {
+      d.origin[t] = ok ? sg.origin[t] : 0.0;
+      d.widths[t] = ok ? sg.widths[t] : 1.0;
+    }
+    d.ortho = ok ? sg.ortho : 1;
+    d.cell_min = ok ? sp.cell_min : 1.0;
+    d.nx = ok ? sp.nx : 1;
+    d.ny = ok ? sp.ny : 1;
+    d.nz = ok ? sp.nz : 1;
+    d.max_reach = ok ? sp.max_reach : 1;
+    d.r_safe = ok ? safe_reach(sp) : 0;
+    d.pt_base = f * n;
+    d.cell_base = f * nc;
+    d.skip = ok ? 0 : 1;
+    out[f] = d;
+  }
 }
 
 // ---------------------------------------------------------------- scan
@@ -2726,6 +2794,7 @@ hipStream_t Engine::stream() {
 
 void Engine::sync() {
   if (stream_ != nullptr) check(hipStreamSynchronize(stream_), "stream wait");
+  check_dcell_status();
 }
 
 void Engine::grow(Buf& b, size_t bytes) {
@@ -3250,6 +3319,133 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
     timings_pending_ = true;
   }
   if (wait && !capturing_) sync();
+}
+
+void Engine::run_dcell(const FrameSpec& spec, int n_frames, const double* cell,
+                       int cell_n, double hint, bool wait) {
+  if (n_frames <= 0) return;
+  hipStream_t st = stream();
+  const int F = n_frames;
+  const long long n = spec.n;
+  const long long ntot = n * F;
+  const long long ctot = dcell_cap_ * F;
+  if (ntot >= INT_MAX / 2 || ctot >= INT_MAX / 2) {
+    throw std::runtime_error("batch exceeds 2^30 points or cells");
+  }
+  // an unread status of an earlier search is reported first
+  if (dcell_pending_) sync();
+  last_F_ = F;
+  last_ntot_ = ntot;
+  last_ctot_ = ctot;
+  const int kc = mode_ == WalkMode::kExactOnly ? 0 : fast_bucket(spec.k);
+  const Sizes z = sizes_of(F, ntot, ctot, std::max(1, spec.k), 512ll << 20);
+  {
+    size_t bytes[kNumBufs];
+    buffer_bytes(z, bytes);
+    bytes[5] = sizeof(int) * 8;  // counters: the plan status rides in slot 4
+    Buf* own[kNumBufs] = {&frames_,  &fpt_,   &fcb_,      &count_,    &status_, &counters_,
+                          &off_,     &tile_off_, &item_off_, &items_,   &cellslot_,
+                          &tmp_key_, &cand_,  &scell_,    &byidx_,    &tbox_,   &tile_cell_,
+                          &cbox_,    &defer_, &flags_,    &heap_d2_,  &heap_idx_};
+    for (int i = 0; i < kNumBufs; ++i) grow(*own[i], bytes[i]);
+  }
+  Bufs B;
+  void** slot = reinterpret_cast<void**>(&B);
+  Buf* own[kNumBufs] = {&frames_,  &fpt_,   &fcb_,      &count_,    &status_, &counters_,
+                        &off_,     &tile_off_, &item_off_, &items_,   &cellslot_,
+                        &tmp_key_, &cand_,  &scell_,    &byidx_,    &tbox_,   &tile_cell_,
+                        &cbox_,    &defer_, &flags_,    &heap_d2_,  &heap_idx_};
+  for (int i = 0; i < kNumBufs; ++i) slot[i] = own[i]->p;
+  FrameDesc d;
+  std::memset(&d, 0, sizeof d);
+  d.xyz = spec.xyz;
+  d.mask = spec.mask;
+  d.out_nn = spec.out_nn;
+  d.out_d2 = spec.out_d2;
+  d.out_reach = spec.out_reach;
+  d.n = spec.n;
+  d.k = spec.k;
+  d.kc = kc;
+  k_plan_dcell<<<1, 64, 0, st>>>(cell, cell_n, hint, d, F, n, spec.k, dcell_cap_, B.frames,
+                                 B.fpt, B.fcb, B.counters + 4);
+  check(hipGetLastError(), "plan launch");
+  const bool timing = timing_ && !capturing_;
+  if (timing) {
+    ev_ = ring_[ring_pos_ % kTimingRing];
+    for (int e = 0; e < 10; ++e) {
+      if (ev_[e] == nullptr) check(hipEventCreate(&ev_[e]), "hipEventCreate");
+    }
+  }
+  hipEvent_t* ev = timing ? ev_ : nullptr;
+  launch_prep(z, B, st, ev);
+  bool any_fast[33] = {};
+  if (kc) any_fast[kc] = true;
+  std::vector<FrameSpec> one(1, spec);
+  launch_walk(z, B, walk_args(z, B, dense_min_for(one)), any_fast, kc == 0, st, ev);
+  if (timing) {
+    ring_pos_ += 1;
+    timings_pending_ = true;
+  }
+  dcell_pending_ = true;
+  if (wait) sync();
+}
+
+void Engine::check_dcell_status() {
+  if (!dcell_pending_) return;
+  dcell_pending_ = false;
+  int stv = 0;
+  check(hipMemcpy(&stv, as<int>(counters_.p) + 4, sizeof(int), hipMemcpyDeviceToHost),
+        "DtoH plan status");
+  if (stv == 1) throw std::runtime_error("bad cell");
+  if (stv == 2) throw std::runtime_error("too many cells");
+}
+
+void Engine::dcell_plan(const double* cell, int cell_n, double hint, double* out) {
+  hipStream_t st = stream();
+  sync();
+  void* dbuf = nullptr;
+  const size_t bytes = sizeof(FrameDesc) + 2 * sizeof(long long) * 2 + sizeof(int) * 2;
+  check(hipMalloc(&dbuf, bytes), "hipMalloc");
+  FrameDesc d;
+  std::memset(&d, 0, sizeof d);
+  FrameDesc got;
+  int stv = 0;
+  long long fcb[2] = {0, 0};
+  try {
+    char* b = static_cast<char*>(dbuf);
+    FrameDesc* df = reinterpret_cast<FrameDesc*>(b);
+    long long* dfpt = reinterpret_cast<long long*>(b + sizeof(FrameDesc));
+    long long* dfcb = dfpt + 2;
+    int* dst = reinterpret_cast<int*>(dfcb + 2);
+    k_plan_dcell<<<1, 64, 0, st>>>(cell, cell_n, hint, d, 1, 0, 0, kMaxCells, df, dfpt, dfcb,
+                                   dst);
+    check(hipGetLastError(), "plan launch");
+    check(hipMemcpyAsync(&got, df, sizeof got, hipMemcpyDeviceToHost, st), "DtoH plan");
+    check(hipMemcpyAsync(fcb, dfcb, sizeof fcb, hipMemcpyDeviceToHost, st), "DtoH plan");
+    check(hipMemcpyAsync(&stv, dst, sizeof stv, hipMemcpyDeviceToHost, st), "DtoH plan");
+    check(hipStreamSynchronize(st), "stream wait");
+  } catch (...) {
+    (void)hipFree(dbuf);
+    throw;
+  }
+  check(hipFree(dbuf), "hipFree");
+  for (int t = 0; t < 9; ++t) {
+    out[t] = got.h[t];
+    out[9 + t] = got.hinv[t];
+  }
+  for (int t = 0; t < 3; ++t) {
+    out[18 + t] = got.origin[t];
+    out[21 + t] = got.widths[t];
+  }
+  out[24] = got.ortho;
+  out[25] = got.cell_min;
+  out[26] = got.nx;
+  out[27] = got.ny;
+  out[28] = got.nz;
+  out[29] = static_cast<double>(fcb[1]);
+  out[30] = got.max_reach;
+  out[31] = stv;
+  for (int t = 32; t < 36; ++t) out[t] = 0.0;
 }
 
 // ------------------------------------------------ fixed-scratch search

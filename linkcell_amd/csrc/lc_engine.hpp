@@ -40,6 +40,7 @@ struct FrameDesc {
   int r_safe;  // largest reach free of stencil wrap (see safe_reach)
   int ortho;
   int kc;  // register-heap bucket for the fast walk, 0 = exact walk only
+  int skip;  // 1: the frame's device-side plan failed; every point counts as masked
 };
 
 // Where the source tiles of a search live (device pointers of the engine);
@@ -96,9 +97,29 @@ class Engine {
   Engine& operator=(const Engine&) = delete;
 
   hipStream_t stream();
+  // Waits for the stream; reports a pending device-cell plan failure.
   void sync();
   // Runs a batch. Throws std::runtime_error on HIP failures.
   void run(const std::vector<FrameSpec>& frames, bool wait);
+  // n_frames frames sharing a device-resident cell of cell_n doubles (3
+  // ortho lengths, 9 rows a b c, 12 with the origin): the cell is inverted
+  // and the mesh planned on the device, with the same arithmetic as the
+  // host plan, so nothing is read back and the sequence has no host copy
+  // (a graph can capture it once buffers are sized). Buffers are sized for
+  // dcell_capacity() cells per frame. A bad cell or a plan above that
+  // capacity makes every point of the search count as masked (-1 / NaN
+  // rows) and is reported as "bad cell" / "too many cells" after a wait
+  // (at once when wait is set, else by the next sync()). spec gives the
+  // points, outputs, mask, n and k of frame 0; frame f follows at strides
+  // n points and n * k outputs.
+  void run_dcell(const FrameSpec& spec, int n_frames, const double* cell, int cell_n,
+                 double hint, bool wait);
+  void set_dcell_capacity(long long cells) { dcell_cap_ = cells < 1 ? 1 : cells; }
+  long long dcell_capacity() const { return dcell_cap_; }
+  // Diagnostics: the device's plan of a device cell as 36 doubles (H by
+  // columns, Hinv, origin, widths, ortho, cell_min, nx, ny, nz, ncell,
+  // max_reach, status); waits.
+  void dcell_plan(const double* cell, int cell_n, double hint, double* out36);
 
   // hipGraph: run the batch once (sizing every buffer), then capture the
   // same batch into a graph that replay() relaunches. The frames' device
@@ -185,6 +206,11 @@ class Engine {
       byidx_, tbox_, cbox_, defer_, heap_d2_, heap_idx_, flags_, stats_,
       sched_, sched_cnt_, tile_cell_;
   int sched_mode_ = 0;
+  // cells per frame the device-cell path sizes its buffers for (the
+  // reference's device limit, src/gpu/knearest.cpp:23)
+  long long dcell_cap_ = 262144;
+  bool dcell_pending_ = false;  // a device-cell search's status is unread
+  void check_dcell_status();
   const float* given_pred_ = nullptr;
   long long given_pred_n_ = 0;
   const float* given_tile_ = nullptr;
