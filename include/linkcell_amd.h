@@ -121,6 +121,18 @@ int lc_gpu_sched_set_prediction(lc_gpu_workspace *ws, const float *kth_dist,
 int lc_gpu_sched_set_tile_prediction(lc_gpu_workspace *ws, const float *kth_dist,
                                      size_t n_tiles);
 
+/* Per source slot instead (the walk's cell-sorted order, as numbered by
+ * lc_gpu_sched_source_labels: n_sources floats, one per active source):
+ * each source's own threshold cap; takes precedence over the per-tile and
+ * per-cell predictions; NULL clears it. Results never change. */
+int lc_gpu_sched_set_source_prediction(lc_gpu_workspace *ws, const float *kth_dist,
+                                       size_t n_sources);
+
+/* For every active source of the last search in the walk's slot order, its
+ * final k-th-neighbour distance rounded up to float (+inf without one): a
+ * device array of at least as many floats as active sources. Waits. */
+int lc_gpu_sched_source_labels(lc_gpu_workspace *ws, float *out, size_t n);
+
 /* Number of source tiles of the last search (waits for it). */
 int lc_gpu_sched_tile_count(lc_gpu_workspace *ws, size_t *n_tiles);
 

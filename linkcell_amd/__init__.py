@@ -179,6 +179,18 @@ class Workspace:
         self._check(self._lib.lc_gpu_sched_set_tile_prediction(
             self._ws, C.c_void_p(kth_ptr or None), int(n_tiles)))
 
+    def sched_set_source_prediction(self, kth_ptr: int, n_sources: int):
+        """Per-source k-th-neighbour distances (device float32 array, the
+        walk's slot order) for SCHED_GIVEN; 0 clears."""
+        self._check(self._lib.lc_gpu_sched_set_source_prediction(
+            self._ws, C.c_void_p(kth_ptr or None), int(n_sources)))
+
+    def source_labels(self, out_ptr: int, n: int):
+        """Final k-th-neighbour distance per source slot of the last search
+        into a device float32 array."""
+        self._check(self._lib.lc_gpu_sched_source_labels(self._ws, C.c_void_p(out_ptr),
+                                                         int(n)))
+
     def tile_count(self) -> int:
         """Source tiles of the last search."""
         n = C.c_size_t(0)

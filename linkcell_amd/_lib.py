@@ -34,6 +34,7 @@ EXPORTS = (
     "lc_gpu_last_timing_ex", "lc_gpu_sched_dump", "lc_gpu_timing_history",
     "lc_gpu_knearest_noalloc", "lc_gpu_noalloc_scratch_bytes",
     "lc_gpu_set_dcell_capacity", "lc_gpu_dcell_plan",
+    "lc_gpu_sched_set_source_prediction", "lc_gpu_sched_source_labels",
 )
 
 
@@ -112,6 +113,8 @@ _PROTOS = {
                                           C.c_double, _vp, _vp, _vp]),
     "lc_gpu_noalloc_scratch_bytes": (_sz, []),
     "lc_gpu_set_dcell_capacity": (C.c_int, [_vp, _sz]),
+    "lc_gpu_sched_set_source_prediction": (C.c_int, [_vp, _vp, _sz]),
+    "lc_gpu_sched_source_labels": (C.c_int, [_vp, _vp, _sz]),
     "lc_gpu_dcell_plan": (C.c_int, [_vp, _vp, C.c_int, C.c_double, _dp]),
     "lc_gpu_last_timing_ex": (C.c_int, [_vp, _dp, C.c_int]),
     "lc_gpu_sched_collect": (C.c_int, [_vp, C.POINTER(_sz)]),

@@ -640,6 +640,32 @@ int lc_gpu_knearest_noalloc(const double* xyz, size_t n, size_t n_frames,
 
 size_t lc_gpu_noalloc_scratch_bytes(void) { return lcamd::kNoAllocScratchBytes; }
 
+int lc_gpu_sched_set_source_prediction(lc_gpu_workspace* ws, const float* kth_dist,
+                                       size_t n_sources) {
+  if (ws == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  ws->ws.impl()->engine.set_given_source_prediction(kth_dist,
+                                                    static_cast<long long>(n_sources));
+  set_gpu_err(nullptr);
+  return 0;
+}
+
+int lc_gpu_sched_source_labels(lc_gpu_workspace* ws, float* out, size_t n) {
+  if (ws == nullptr || out == nullptr) {
+    set_gpu_err("null pointer");
+    return 1;
+  }
+  return guarded([&] {
+    try {
+      ws->ws.impl()->engine.source_labels(out, static_cast<long long>(n));
+    } catch (const std::runtime_error& e) {
+      throw linkcell::Error(e.what());
+    }
+  });
+}
+
 int lc_gpu_set_dcell_capacity(lc_gpu_workspace* ws, size_t cells_per_frame) {
   if (ws == nullptr) {
     set_gpu_err("null pointer");

@@ -1056,6 +1056,10 @@ struct WalkArgs {
   const float* pred;
   const float* pred_tile;
   long long pred_tile_n;  // tiles beyond it are not capped
+  // optional predicted k-th-neighbour distance per source slot (the
+  // walk's cell-sorted order): the source's own threshold cap
+  const float* pred_src;
+  long long pred_src_n;
   // capped row walk: rows walked a second time without their cap are
   // counted here (null: not counted)
   int* retry_count;
@@ -1634,6 +1638,19 @@ __device__ __forceinline__ void surface_cell(int r, int i, int& dx, int& dy,
   dz = -r + i % s;
 }
 
+// i-th tile of a cell of nt tiles in order of distance from tile tl in the
+// cell's sub-cell order: tl, tl + 1, tl - 1, tl + 2, ... while both sides
+// last, then the longer side (a bijection of 0 .. nt - 1).
+#ifndef LC_HOME_SPIRAL
+#define LC_HOME_SPIRAL 1
+#endif
+__device__ __forceinline__ int spiral_tile(int i, int tl, int nt) {
+  const int left = tl, right = nt - 1 - tl;
+  const int a = left < right ? left : right;
+  if (i <= 2 * a) return tl + ((i & 1) ? (i + 1) / 2 : -(i / 2));
+  return right > left ? tl + (i - a) : tl - (i - a);
+}
+
 // First wave of frame f in the row walk: frames start on wave boundaries
 // so that every wave's frame data is uniform. ceil(t/4) + f leaves at
 // most one spare wave per frame and needs no scan.
@@ -1818,6 +1835,10 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   }
   if (A.pred_tile != nullptr && T < A.pred_tile_n) {
     const double q = static_cast<double>(A.pred_tile[T]);
+    thr_cap = fmin(thr_cap, q * q);
+  }
+  if (A.pred_src != nullptr && s < A.pred_src_n) {
+    const double q = static_cast<double>(A.pred_src[s]);
     thr_cap = fmin(thr_cap, q * q);
   }
   bool live = valid;
@@ -2140,9 +2161,13 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
             const int ot0 = __shfl(ct0, rbase + own);
             const int ocode = __shfl(ccode, rbase + own);
             const int oex = __shfl(cpre - cnt_c, rbase + own);
-            const int t = p - oex;
-            const bool opening = r == 1 && cc == 16 && own == 0 &&
-                                 (t == tl || t == tl_nb);
+            // the home cell's tiles go outward from the row's own tile in
+            // sub-cell order (tl, tl + 1, tl - 1, ...): the nearest first,
+            // so the thresholds tighten before the far tiles are tested
+            const bool home = r == 1 && cc == 16 && own == 0;
+            const int t = (LC_HOME_SPIRAL && home) ? spiral_tile(p - oex, tl, nt_home)
+                                                   : p - oex;
+            const bool opening = home && (t == tl || t == tl_nb);
             bool need = false;
             double bx0 = 0, bx1 = 0, by0 = 0, by1 = 0, bz0 = 0, bz1 = 0;
             if (pv && !opening) {
@@ -2376,6 +2401,25 @@ __global__ __launch_bounds__(256) void k_tile_labels(
     if (kth == kth) v = __double2float_ru(sqrt(kth));
   }
   atomicMax(&out[T], __float_as_int(v));
+}
+
+// Final k-th-neighbour distance of every source slot of the last search
+// (rounded up to float, +inf without one): per-source schedule labels.
+__global__ __launch_bounds__(256) void k_src_labels(
+    const FrameDesc* __restrict__ fr, const long long* __restrict__ fcb, int F,
+    const Cand* __restrict__ cand, const int* __restrict__ scell,
+    const int* __restrict__ off, long long ncells, float* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= off[ncells]) return;
+  const int g = scell[s];
+  const FrameDesc& d = fr[find_frame(fcb, F, g)];
+  float v = __builtin_inff();
+  if (d.out_d2 != nullptr) {
+    const int i = static_cast<int>(cand[s].id);
+    const double kth = d.out_d2[static_cast<size_t>(i) * d.k + d.k - 1];
+    if (kth == kth) v = __double2float_ru(sqrt(kth));
+  }
+  out[s] = v;
 }
 
 // ---------------------------------------------------------- exact walk
@@ -3284,7 +3328,11 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
     wa.pred_tile = as<float>(pred_tile_.p);
     wa.pred_tile_n = tcap;
   }
-  if (sched_mode_ == 2) {
+  if (sched_mode_ == 2 && given_src_ != nullptr) {
+    wa.pred_src = given_src_;
+    wa.pred_src_n = given_src_n_;
+  }
+  if (sched_mode_ == 2 && given_src_ == nullptr) {
     if (given_tile_ != nullptr) {
       wa.pred_tile = given_tile_;
       wa.pred_tile_n = given_tile_n_;
@@ -3691,6 +3739,21 @@ void Engine::sched_dump(unsigned short* x_collect, unsigned short* x_predict,
   }
   check(hipFree(dx), "hipFree");
   check(hipFree(dp), "hipFree");
+}
+
+void Engine::source_labels(float* out, long long n) {
+  sync();
+  if (last_F_ == 0) return;
+  int ns = 0;
+  check(hipMemcpy(&ns, as<int>(off_.p) + last_ctot_, sizeof(int), hipMemcpyDeviceToHost),
+        "DtoH source count");
+  if (n < ns) throw std::runtime_error("label buffer shorter than the search's sources");
+  hipStream_t st = stream();
+  k_src_labels<<<blocks_for(last_ntot_, 256), 256, 0, st>>>(
+      as<FrameDesc>(frames_.p), as<long long>(fcb_.p), last_F_, as<Cand>(cand_.p),
+      as<int>(scell_.p), as<int>(off_.p), last_ctot_, out);
+  check(hipGetLastError(), "source label launch");
+  sync();
 }
 
 long long Engine::tile_count() {

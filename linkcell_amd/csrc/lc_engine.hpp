@@ -167,6 +167,16 @@ class Engine {
     given_tile_ = kth;
     given_tile_n_ = n;
   }
+  // Per source slot (the walk's cell-sorted order, see source_labels);
+  // takes precedence over both; null clears it.
+  void set_given_source_prediction(const float* kth, long long n) {
+    given_src_ = kth;
+    given_src_n_ = n;
+  }
+  // For every source slot of the last search, its final k-th-neighbour
+  // distance rounded up to float (+inf without one), into the device
+  // array out (at least as many floats as the search's active sources).
+  void source_labels(float* out, long long n);
   // Source tiles (16 consecutive slots of one home cell in the walk's
   // cell-sorted order) of the last search; waits for it.
   long long tile_count();
@@ -215,6 +225,8 @@ class Engine {
   long long given_pred_n_ = 0;
   const float* given_tile_ = nullptr;
   long long given_tile_n_ = 0;
+  const float* given_src_ = nullptr;
+  long long given_src_n_ = 0;
   int last_F_ = 0;
   long long last_ntot_ = 0, last_ctot_ = 0;
   std::unique_ptr<Predictor> pred_;
