@@ -245,17 +245,25 @@ def bench_batch(args):
                                       quantile=args.quantile)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
+        def same_as_plain():
+            got = keep[1::3] + keep[2::3]
+            return all(bool(torch.equal(g.view(torch.int64) if g.dtype == torch.float64 else g,
+                                        r.view(torch.int64) if r.dtype == torch.float64 else r))
+                       for g, r in zip(got, ref))
+
+        ws.set_schedule(3)
+        ms_d = timed_graph("learned depth graph")
+        if not same_as_plain():
+            raise SystemExit("bench: the depth schedule differs from the plain walk")
         ws.set_schedule(1)
         ms_l = timed_graph("learned graph")
-        got = keep[1::3] + keep[2::3]
-        same = all(bool(torch.equal(g.view(torch.int64) if g.dtype == torch.float64 else g,
-                                    r.view(torch.int64) if r.dtype == torch.float64 else r))
-                   for g, r in zip(got, ref))
+        same = same_as_plain()
         if not same:
             raise SystemExit("bench: the learned walk differs from the plain walk")
         learned = {"ms_per_step": ms_l, "train_batches": 4, "samples": ws.sched_samples(),
                    "loss": loss, "margin_log2": margin, "train_s": t1 - t0,
-                   "bitwise_equal_to_plain": same}
+                   "bitwise_equal_to_plain": same,
+                   "depth_schedule": {"ms_per_step": ms_d, "over_plain": ms_plain / ms_d}}
         ms_step, sched = ms_l, "learned"
     out = {
         "metric": "source-queries/sec, 256 uneven frames in one captured pipeline",
@@ -446,12 +454,26 @@ def main():
         retried = ws.last_retried()
         ws.set_timing(False)
         ntiles = ws.tile_count()
+        # the depth schedule on the same network: shells below each tile's
+        # predicted depth listed in one pass, tiles packed by depth
+        ws.set_schedule(3)
+        step(wait=True)
+        same_d = bool(torch.equal(nn, ref_nn)) and bool(
+            torch.equal(d2.view(torch.int64), ref_d2.view(torch.int64)))
+        if not same_d:
+            raise SystemExit("bench: the depth schedule differs from the plain walk")
+        ms_d, kern_d, work_d = leg("learned depth")
+        ws.set_schedule(1)
+        depth = {"ms_per_step": ms_d, "walk_ms": kern_d["walk_exact"], "kernels_ms": kern_d,
+                 "over_plain": ms_plain / ms_d, "bitwise_equal_to_plain": same_d,
+                 "row_rounds": work_d["row_tiles"], "row_useful": work_d["row_useful"],
+                 "merges": work_d["row_flushes"], "accepts": work_d["row_accepts"]}
         learned = {"ms_per_step": ms_l, "kernels_ms": kern_l, "walk_work": work_l,
                    "train_frames": train_frames, "samples": samples,
                    "epochs": args.epochs, "loss": loss, "margin_log2": margin,
                    "quantile": args.quantile, "collect_s": t1 - t0, "train_s": t2 - t1,
                    "retried_rows": retried, "rows": ntiles,
-                   "bitwise_equal_to_plain": same}
+                   "bitwise_equal_to_plain": same, "depth_schedule": depth}
 
     if learned is not None:
         ms_step, kern, work, sched = learned["ms_per_step"], learned["kernels_ms"], \

@@ -103,6 +103,13 @@ int lc_gpu_set_dense_min(lc_gpu_workspace *ws, int dense_min);
 #define LC_GPU_SCHED_CELL 0
 #define LC_GPU_SCHED_LEARNED 1
 #define LC_GPU_SCHED_GIVEN 2
+/* The learned caps plus the depth schedule: each tile's predicted depth
+ * R = ceil(cap / cell_min) (at most the wrap-free reach); the row lists the
+ * cells of shells 1 .. R in one pass and the exact stop test decides from
+ * shell R on; the tiles of every 16 are packed by R so that rows of one
+ * wave need about the same number of shells. Where a per-source reach is
+ * asked for (lc_gpu_knearest_ex), rows walk shell by shell. */
+#define LC_GPU_SCHED_LEARNED_DEPTH 3
 int lc_gpu_set_schedule(lc_gpu_workspace *ws, int mode);
 
 /* Predicted k-th-neighbour distance per home cell for LC_GPU_SCHED_GIVEN:

@@ -161,7 +161,7 @@ class Workspace:
     def set_dense_min(self, v: int):
         self._check(self._lib.lc_gpu_set_dense_min(self._ws, int(v)))
 
-    SCHED_CELL, SCHED_LEARNED, SCHED_GIVEN = 0, 1, 2
+    SCHED_CELL, SCHED_LEARNED, SCHED_GIVEN, SCHED_LEARNED_DEPTH = 0, 1, 2, 3
 
     def set_schedule(self, mode: int):
         """Order of the lane-per-source walk (results never change)."""

@@ -523,7 +523,7 @@ int lc_gpu_set_schedule(lc_gpu_workspace* ws, int mode) {
     return 1;
   }
   if (mode != LC_GPU_SCHED_CELL && mode != LC_GPU_SCHED_LEARNED &&
-      mode != LC_GPU_SCHED_GIVEN) {
+      mode != LC_GPU_SCHED_GIVEN && mode != LC_GPU_SCHED_LEARNED_DEPTH) {
     set_gpu_err("unknown schedule");
     return 1;
   }

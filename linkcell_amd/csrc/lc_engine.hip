@@ -1063,6 +1063,12 @@ struct WalkArgs {
   // capped row walk: rows walked a second time without their cap are
   // counted here (null: not counted)
   int* retry_count;
+  // depth schedule of the row walk (learned): rows list the cells of
+  // shells 1 .. R in one pass, R = ceil(cap / cell_min) of the tile's
+  // cap, and the stop test decides from shell R on; tile_perm (optional)
+  // packs the tiles of every 16 by that depth
+  int depth;
+  const int* tile_perm;
 };
 
 enum {
@@ -1698,7 +1704,9 @@ struct RowOcc {
 #else
 #define LC_ROW_BLOCKS(KC) RowOcc<KC>::kMinBlocks
 #endif
-template <int KC, bool kStats>
+// kDepth: the depth schedule (rows list shells 1 .. predicted depth in one
+// pass); without it every row of a wave is in the same shell.
+template <int KC, bool kStats, bool kDepth>
 __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A) {
   constexpr int W = 4;   // waves per block
   constexpr int BC = 8;  // buffered candidates per lane between merges
@@ -1743,7 +1751,8 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   const int tf = (wg - row_wave0(A.tile_off, A.fcb, f)) * 4 + rw;
   if (__ballot(tf < nt_frame) == 0) return;
   const bool has_tile = tf < nt_frame;
-  const int T = tb + (has_tile ? tf : 0);
+  int T = tb + (has_tile ? tf : 0);
+  if (A.tile_perm != nullptr && has_tile) T = A.tile_perm[T];
   const int g = A.tile_cell[T];
   const int o0 = A.off[g];
   const int cnt = A.off[g + 1] - o0;
@@ -1840,6 +1849,14 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   if (A.pred_src != nullptr && s < A.pred_src_n) {
     const double q = static_cast<double>(A.pred_src[s]);
     thr_cap = fmin(thr_cap, q * q);
+  }
+  // predicted depth of the row (1 without a depth schedule, or where the
+  // per-source reach is asked for: the reach is the reference's stop
+  // shell, which only a shell-by-shell walk observes)
+  int rp = 1;
+  if (kDepth && A.pred_tile != nullptr && T < A.pred_tile_n && d.out_reach == nullptr) {
+    const double q = ceil(static_cast<double>(A.pred_tile[T]) / d.cell_min);
+    rp = q >= static_cast<double>(d.r_safe) ? d.r_safe : (q > 1.0 ? static_cast<int>(q) : 1);
   }
   bool live = valid;
   double thr = live ? thr_cap : -inf;  // -inf: finished (never accepts)
@@ -2076,11 +2093,14 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     if (kStats) cyc_drain += __builtin_readcyclecounter() - t0;
   };
 
+  // the row's shell (rows of one wave may be in different shells under a
+  // depth schedule)
+  int r = 1;
 #pragma unroll 1
-  for (int r = 1;; ++r) {
-    const int ncr = shell_cells(r);
+  for (;; ++r) {
+    int ncr = shell_cells(r);
     cur_r = r;
-    if (kStats && r == 2 && attempt == 0) {
+    if (kStats && !st_wave2 && __ballot(r >= 2) && attempt == 0) {
       st_wave2 = 1;
       cyc_r2 = __builtin_readcyclecounter();
     }
@@ -2098,6 +2118,13 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
         if (!__ballot(fill)) break;
         if (fill) {
           if (p0 >= total) {
+            // below the predicted depth the next shell's cells follow in
+            // the same pass, with no stop test between
+            if (kDepth && cc >= ncr && r < rp) {
+              r += 1;
+              ncr = shell_cells(r);
+              cc = 0;
+            }
             if (cc >= ncr) {
               more = false;
             } else {
@@ -2267,6 +2294,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     top_init<KC>(h, k);
     min_rej = inf;
     thr_cap = inf;
+    rp = 1;
     live = valid;
     thr = live ? inf : -inf;
     done_reach = 0;
@@ -2401,6 +2429,39 @@ __global__ __launch_bounds__(256) void k_tile_labels(
     if (kth == kth) v = __double2float_ru(sqrt(kth));
   }
   atomicMax(&out[T], __float_as_int(v));
+}
+
+// Predicted depth of a tile under the depth schedule: the first shell whose
+// stop bound covers the tile's cap (1 .. r_safe).
+__device__ __forceinline__ int cap_depth(float cap, const FrameDesc& d) {
+  const double q = ceil(static_cast<double>(cap) / d.cell_min);
+  return q >= static_cast<double>(d.r_safe) ? d.r_safe : (q > 1.0 ? static_cast<int>(q) : 1);
+}
+
+// Depth packing of the row walk: the tiles of every 16 (counted from the
+// frame's first tile, so each wave's four stay in one group and frame) in
+// order of predicted depth, then tile: perm[first + rank] = tile.
+#ifndef LC_DEPTH_PACK
+#define LC_DEPTH_PACK 1
+#endif
+__global__ __launch_bounds__(256) void k_depth_order(
+    const FrameDesc* __restrict__ fr, const long long* __restrict__ fcb, int F,
+    const int* __restrict__ tile_off, const int* __restrict__ tile_cell, long long ncells,
+    const float* __restrict__ cap, int* __restrict__ perm) {
+  const int T = blockIdx.x * blockDim.x + threadIdx.x;
+  if (T >= tile_off[ncells]) return;
+  const int f = find_frame(fcb, F, tile_cell[T]);
+  const FrameDesc& d = fr[f];
+  const int tb = tile_off[fcb[f]], te = tile_off[fcb[f + 1]];
+  const int g0 = tb + ((T - tb) & ~15);
+  const int g1 = g0 + 16 < te ? g0 + 16 : te;
+  const int my = cap_depth(cap[T], d);
+  int rank = 0;
+  for (int u = g0; u < g1; ++u) {
+    const int du = cap_depth(cap[u], d);
+    rank += (du < my || (du == my && u < T)) ? 1 : 0;
+  }
+  perm[g0 + rank] = T;
 }
 
 // Final k-th-neighbour distance of every source slot of the last search
@@ -2860,7 +2921,7 @@ void Engine::grow(Buf& b, size_t bytes) {
 }
 
 void Engine::free_all() {
-  Buf* all[] = {&frames_,   &fpt_,      &fcb_,    &count_,  &status_,
+  Buf* all[] = {&tile_perm_, &frames_,   &fpt_,      &fcb_,    &count_,  &status_,
                 &counters_, &off_,      &tile_off_, &item_off_, &items_,
                 &cellslot_, &tmp_key_,  &cand_,
                 &scell_,    &byidx_,  &tbox_,   &cbox_,
@@ -3119,9 +3180,11 @@ void launch_walk(const Sizes& z, const Bufs& B, const WalkArgs& wa,
 #define LC_WALK(KC)                                                             \
   if (any_fast[KC] && row_mode) {                                               \
     if (wa.stats != nullptr) {                                                  \
-      k_walk_rows<KC, true><<<row_blocks, 256, 0, st>>>(wa);                    \
+      if (wa.depth) k_walk_rows<KC, true, true><<<row_blocks, 256, 0, st>>>(wa);  \
+      else k_walk_rows<KC, true, false><<<row_blocks, 256, 0, st>>>(wa);         \
     } else {                                                                    \
-      k_walk_rows<KC, false><<<row_blocks, 256, 0, st>>>(wa);                   \
+      if (wa.depth) k_walk_rows<KC, false, true><<<row_blocks, 256, 0, st>>>(wa); \
+      else k_walk_rows<KC, false, false><<<row_blocks, 256, 0, st>>>(wa);        \
     }                                                                           \
     check(hipGetLastError(), "walk launch");                                    \
   } else if (any_fast[KC]) {                                                    \
@@ -3320,13 +3383,27 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
     wa.stats = as<unsigned long long>(stats_.p);
   }
   const float* pred = nullptr;
-  if (sched_mode_ == 1 && pred_ != nullptr && pred_->trained() && row_mode) {
+  if ((sched_mode_ == 1 || sched_mode_ == 3) && pred_ != nullptr && pred_->trained() &&
+      row_mode) {
     // learned caps of every source tile, from the counts and tile boxes
     // just built
     grow(pred_tile_, sizeof(float) * static_cast<size_t>(tcap));
     pred_->predict_caps(tile_src(ctot, F), tcap, as<float>(pred_tile_.p), st);
     wa.pred_tile = as<float>(pred_tile_.p);
     wa.pred_tile_n = tcap;
+    if (sched_mode_ == 3) {
+      // depth schedule: shells below each tile's predicted depth in one
+      // pass, tiles packed by that depth
+      wa.depth = 1;
+      if (LC_DEPTH_PACK) {
+        grow(tile_perm_, sizeof(int) * static_cast<size_t>(tcap));
+        k_depth_order<<<blocks_for(tcap, 256), 256, 0, st>>>(
+            B.frames, B.fcb, F, B.tile_off, B.tile_cell, ctot, as<float>(pred_tile_.p),
+            as<int>(tile_perm_.p));
+        check(hipGetLastError(), "depth order launch");
+        wa.tile_perm = as<int>(tile_perm_.p);
+      }
+    }
   }
   if (sched_mode_ == 2 && given_src_ != nullptr) {
     wa.pred_src = given_src_;

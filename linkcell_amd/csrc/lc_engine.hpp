@@ -148,7 +148,10 @@ class Engine {
   void set_dense_min(int v) { dense_min_override_ = v < -1 ? 0 : v; }
   int dense_min_for(const std::vector<FrameSpec>& frames) const;
   // Walk schedule: 0 = plain; 1 = the learned predictor's k-th-neighbour
-  // distance per home cell; 2 = a caller-given distance per cell (device
+  // distance per source tile as the row walk's threshold cap; 3 = the
+  // same caps plus the depth schedule (each row lists the shells below its
+  // predicted depth in one pass, rows packed by predicted depth); 2 = a
+  // caller-given distance per cell (device
   // array of floats, one per cell of the batch in frame order). The
   // distance seeds the walk's pruning threshold (checked exactly at every
   // shell end; a source it does not cover is finished by the exact walk)
@@ -230,7 +233,7 @@ class Engine {
   int last_F_ = 0;
   long long last_ntot_ = 0, last_ctot_ = 0;
   std::unique_ptr<Predictor> pred_;
-  Buf pred_tile_, labels_;
+  Buf pred_tile_, labels_, tile_perm_;
   TileSrc tile_src(long long ctot, int F);
   void* staging_ = nullptr;
   size_t staging_bytes_ = 0;

@@ -20,7 +20,7 @@ from oracle import mlp
 
 pytestmark = pytest.mark.gpu
 
-LEARNED, GIVEN, PLAIN = 1, 2, 0
+LEARNED, GIVEN, PLAIN, DEPTH = 1, 2, 0, 3
 
 
 @pytest.fixture(scope="module")
@@ -196,6 +196,85 @@ def test_trained_network_saves_work(ws, trained):
     assert retried < 0.05 * ntiles
     assert learned["row_useful"] < untrained["row_useful"]
     assert learned["row_useful"] < 1.02 * plain["row_useful"]
+
+
+@pytest.mark.parametrize("margin", [None, 3.0])
+def test_depth_schedule_is_bitwise_the_plain_walk(ws, trained, margin):
+    """Rows list shells 1 .. predicted depth in one pass and are packed by
+    that depth; with the trained margin most rows keep depth 1, with a
+    margin of 3 (caps 8x) every row walks more shells than it needs, up to
+    the wrap-free reach: the results stay the plain walk's."""
+    s = _Search(ws, wl.clustered_c3_small(0))
+    ws.set_schedule(PLAIN)
+    pn, pd = s.run()
+    if margin is not None:
+        ws.sched_set_params(trained["params"], margin)
+    ws.set_schedule(DEPTH)
+    try:
+        dn, dd = s.run()
+        ws.set_timing(True)
+        s.run()
+        retried = ws.last_retried()
+        ws.set_timing(False)
+    finally:
+        ws.set_schedule(PLAIN)
+        ws.sched_set_params(trained["params"], trained["margin"])
+    assert np.array_equal(dn, pn) and same_bits(dd, pd)
+    if margin is not None:
+        assert retried == 0  # caps 8x above the prediction never cut a row short
+    rng = np.random.default_rng(1)
+    src = np.sort(rng.choice(s.n, 2000, replace=False))
+    on, od, _, _, _, _ = orc.knearest_sample(s.cfg["xyz"], s.cfg["cell"], 16, src,
+                                             cell_hint=3.0)
+    assert np.array_equal(dn[src], on[src]) and same_bits(dd[src], od[src])
+
+
+@pytest.mark.parametrize("name", ["c2", "c5i", "c4"])
+def test_depth_schedule_on_other_workloads_keeps_results(ws, trained, name):
+    # another density (and, for the batch, frames of other k and cells): the
+    # predicted depths are wrong in both directions
+    if name == "c2":
+        cfgs = [wl.triclinic_c2(n=65536)]
+    elif name == "c5i":
+        cfgs = [dict(fr, name="c5i") for fr in wl.small_c5i(n_frames=3)]
+    else:
+        cfgs = wl.batched_c4(12, seed0=7000)
+    ws.sched_set_params(trained["params"], 1.5)
+    try:
+        for cfg in cfgs:
+            ws.set_schedule(PLAIN)
+            pn, pd = run_device(ws, cfg["xyz"], cfg["cell"], cfg["k"], cfg.get("mask"), 3.0)
+            ws.set_schedule(DEPTH)
+            dn, dd = run_device(ws, cfg["xyz"], cfg["cell"], cfg["k"], cfg.get("mask"), 3.0)
+            ws.set_schedule(PLAIN)
+            assert np.array_equal(dn, pn) and same_bits(dd, pd)
+    finally:
+        ws.set_schedule(PLAIN)
+        ws.sched_set_params(trained["params"], trained["margin"])
+
+
+def test_depth_schedule_in_a_batch_keeps_results(ws, trained):
+    import torch
+    from test_gpu_configs import _device_batch
+    frames = wl.batched_c4(24, seed0=7100)
+    keep, recs = _device_batch(frames)
+    ws.set_schedule(PLAIN)
+    ws.knearest_batch(recs)
+    ref = [(nn.clone(), d2.clone()) for _, nn, d2 in keep]
+    ws.sched_set_params(trained["params"], 2.0)
+    ws.set_schedule(DEPTH)
+    try:
+        for _, nn, d2 in keep:
+            nn.fill_(-5)
+            d2.fill_(-5.0)
+        torch.cuda.synchronize()
+        ws.knearest_batch(recs)
+    finally:
+        ws.set_schedule(PLAIN)
+        ws.sched_set_params(trained["params"], trained["margin"])
+    for f, ((_, nn, d2), (rn, rd)) in enumerate(zip(keep, ref)):
+        assert torch.equal(nn, rn), f
+        assert torch.equal(d2.view(torch.int64), rd.view(torch.int64)), f
 
 
 @pytest.mark.parametrize("name", ["c1", "c2", "c5i"])
