@@ -642,7 +642,7 @@ __global__ __launch_bounds__(256) void k_tile_box(
     const int* __restrict__ items, const int* __restrict__ item_off,
     long long ncells, const int* __restrict__ off,
     const int* __restrict__ tile_off, const Cand* __restrict__ cand,
-    Box* __restrict__ tbox) {
+    Box* __restrict__ tbox, Box* __restrict__ hbox) {
   const int w = uni(static_cast<int>(blockIdx.x * (blockDim.x / kWave) +
                                      threadIdx.x / kWave));
   const int lane = threadIdx.x & (kWave - 1);
@@ -658,21 +658,57 @@ __global__ __launch_bounds__(256) void k_tile_box(
   const double inf = as_f64(kInfBits);
   const Cand cr = cand[valid ? a : o0];
   const double x = cr.x, y = cr.y, z = cr.z;
-  const double lx = row_min16(valid ? x : inf), hx = row_max16(valid ? x : -inf);
-  const double ly = row_min16(valid ? y : inf), hy = row_max16(valid ? y : -inf);
-  const double lz = row_min16(valid ? z : inf), hz = row_max16(valid ? z : -inf);
+  // each half row (8 lanes) reduces, then row_mirror brings the other
+  // half's result: lane 0 of the row holds both halves and the whole tile
+  auto half_min = [](double v) {
+    v = fmin(v, row_dpp<0xB1>(v));
+    v = fmin(v, row_dpp<0x4E>(v));
+    return fmin(v, row_dpp<0x141>(v));
+  };
+  auto half_max = [](double v) {
+    v = fmax(v, row_dpp<0xB1>(v));
+    v = fmax(v, row_dpp<0x4E>(v));
+    return fmax(v, row_dpp<0x141>(v));
+  };
+  const double alx = half_min(valid ? x : inf), ahx = half_max(valid ? x : -inf);
+  const double aly = half_min(valid ? y : inf), ahy = half_max(valid ? y : -inf);
+  const double alz = half_min(valid ? z : inf), ahz = half_max(valid ? z : -inf);
+  const double blx = row_dpp<0x140>(alx), bhx = row_dpp<0x140>(ahx);
+  const double bly = row_dpp<0x140>(aly), bhy = row_dpp<0x140>(ahy);
+  const double blz = row_dpp<0x140>(alz), bhz = row_dpp<0x140>(ahz);
   if ((lane & (kTile - 1)) == 0 && valid) {
     const long long t = t0 + (q * kWave + lane) / kTile;
     Box b;
-    b.lx = lx;
-    b.hx = hx;
-    b.ly = ly;
-    b.hy = hy;
-    b.lz = lz;
-    b.hz = hz;
+    b.lx = fmin(alx, blx);
+    b.hx = fmax(ahx, bhx);
+    b.ly = fmin(aly, bly);
+    b.hy = fmax(ahy, bhy);
+    b.lz = fmin(alz, blz);
+    b.hz = fmax(ahz, bhz);
     b.pad0 = 0.0;
     b.pad1 = 0.0;
     tbox[t] = b;
+    if (hbox != nullptr) {
+      // an empty second half keeps the (+inf, -inf) box: no point is
+      // within any distance of it
+      Box h;
+      h.pad0 = 0.0;
+      h.pad1 = 0.0;
+      h.lx = alx;
+      h.hx = ahx;
+      h.ly = aly;
+      h.hy = ahy;
+      h.lz = alz;
+      h.hz = ahz;
+      hbox[2 * t] = h;
+      h.lx = blx;
+      h.hx = bhx;
+      h.ly = bly;
+      h.hy = bhy;
+      h.lz = blz;
+      h.hz = bhz;
+      hbox[2 * t + 1] = h;
+    }
   }
 }
 
@@ -1050,6 +1086,7 @@ struct WalkArgs {
   const int* item_off;
   long long ncells;
   const Box* tbox;
+  const Box* hbox;  // the two half-tile boxes of tile t at 2t, 2t + 1 (or null)
   const Box* cbox;
   int dense_min;
   int* defer_list;
@@ -1704,6 +1741,30 @@ __device__ __forceinline__ int row_wave0(const int* tile_off,
 #ifndef LC_ROW_SELECT
 #define LC_ROW_SELECT 1
 #endif
+// Heaps of LC_ROW_HALF_KC slots and more list half tiles (8 members, each
+// with its own box) and sweep two per round, lanes 0-7 and 8-15 of the row:
+// a half whose box lies beyond every source's threshold is never swept.
+// The list then holds up to LC_ROW_FILL_HALF - 1 + 32 half tiles. Measured
+// (walk): FCC 8M frame, k = 32, 10.33 ms with whole tiles, 9.79 with half
+// tiles (28% fewer rounds); clustered 4M frame, k = 16, 4.08 ms with whole
+// tiles, 4.23 with half tiles (20 registers spilled at 3 waves per SIMD)
+// and 5.41 with half tiles at 2 waves per SIMD, so the 16-slot heap keeps
+// whole tiles. LC_ROW_HALF 0 turns them off.
+#ifndef LC_ROW_HALF
+#define LC_ROW_HALF 1
+#endif
+#ifndef LC_ROW_HALF_KC
+#define LC_ROW_HALF_KC 32
+#endif
+#ifndef LC_ROW_FILL_HALF
+#define LC_ROW_FILL_HALF 32
+#endif
+// Selection keys held in registers through a drain (read from LDS once, at
+// its start) rather than read from LDS every round: 4.08 against 4.09 ms
+// (clustered 4M), 9.79 against 10.07 (FCC 8M, half tiles).
+#ifndef LC_KEYS_REG
+#define LC_KEYS_REG 1
+#endif
 // Resident 256-thread blocks per CU the register budget is cut for: 3 (3
 // waves per SIMD, 168 VGPRs) for the heaps of up to 16, 2 (256 VGPRs) for
 // the 32-slot heap. Measured on the clustered 4M (k = 16), the uneven batch
@@ -1733,10 +1794,13 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   // pays for the heaps of up to 16 (C3 walk 5.27 ms against 5.40 without)
   // and not for the 32-slot heap (FCC 11.15 ms against 10.99)
   constexpr bool kStepBranch = KC < 32;
-  constexpr int kFill = LC_ROW_FILL;  // list length at which a drain starts
+  constexpr bool kHalf = LC_ROW_HALF != 0 && KC >= LC_ROW_HALF_KC;
+  static_assert(!kHalf || LC_ROW_SELECT, "half tiles are drained by selection");
+  // list length at which a drain starts
+  constexpr int kFill = kHalf ? LC_ROW_FILL_HALF : LC_ROW_FILL;
   // tile list entries per row: a fill round starts below kFill entries and
-  // appends at most 16
-  constexpr int LC = kFill - 1 + 16;
+  // appends at most 16 (32 half tiles)
+  constexpr int LC = kFill - 1 + (kHalf ? 32 : 16);
   static_assert(kFill >= 1 && LC <= 64, "tile list capacity");
   constexpr int NK = (LC + 15) / 16;  // list keys per lane when selecting
   // buffered candidates: one 16-byte record (d2 bits, index) per slot and
@@ -2067,17 +2131,20 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     const unsigned long long t0 = kStats ? __builtin_readcyclecounter() : 0;
     double nxq = nan, nyq = nan, nzq = nan;
     int njq = -1, ncode = 13;
-    auto fetch = [&](int e) LC_INLINE {
+    // half tiles: lanes 0-7 load entry e, lanes 8-15 entry e2
+    auto fetch = [&](int e, int e2) LC_INLINE {
       nxq = nyq = nzq = nan;
       njq = -1;
       ncode = 13;
-      if (e >= 0) {
-        const int2 ent = s_list[wv][rw][e];
+      const int el = (kHalf && li >= 8) ? e2 : e;
+      if (el >= 0) {
+        const int2 ent = s_list[wv][rw][el];
         const int sl0 = ent.x;
         const int meta = ent.y;
         ncode = meta & 31;
-        if (li < (meta >> 5)) {
-          const Cand q = A.cand[sl0 + li];
+        const int m = kHalf ? (li & 7) : li;
+        if (m < (meta >> 5)) {
+          const Cand q = A.cand[sl0 + m];
           nxq = q.x;
           nyq = q.y;
           nzq = q.z;
@@ -2085,7 +2152,60 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
         }
       }
     };
-#if LC_ROW_SELECT
+#if LC_ROW_SELECT && LC_KEYS_REG
+    // nearest first, keys in registers: lane li holds the keys of entries
+    // li, li + 16, ...; taken entries become ~0
+    unsigned kr[NK];
+#pragma unroll
+    for (int j = 0; j < NK; ++j) {
+      const int e = li + 16 * j;
+      kr[j] = e < lcount ? s_key[wv][rw][e] : ~0u;
+    }
+    auto take = [&](unsigned key) LC_INLINE {
+      const int e = static_cast<int>(key & 63u);
+#pragma unroll
+      for (int j = 0; j < NK; ++j) kr[j] = (li == (e & 15) && j == (e >> 4)) ? ~0u : kr[j];
+    };
+    auto beyond = [&](unsigned key) LC_INLINE {
+      return static_cast<double>(__uint_as_float(key & ~63u)) > rthr;
+    };
+    // the row's smallest key (and, for half tiles, the second smallest):
+    // -1 once the list is empty or the nearest bound is beyond the row's
+    // threshold (every later entry is then beyond it too)
+    int e2n = -1;
+    auto next = [&](int) LC_INLINE {
+      unsigned m1 = ~0u, m2 = ~0u;
+#pragma unroll
+      for (int j = 0; j < NK; ++j) {
+        const unsigned v = kr[j];
+        if (kHalf) m2 = min(m2, max(m1, v));
+        m1 = min(m1, v);
+      }
+      auto pair_step = [&](auto ctrl) LC_INLINE {
+        constexpr int c = decltype(ctrl)::value;
+        const unsigned b1 = static_cast<unsigned>(
+            __builtin_amdgcn_update_dpp(0, static_cast<int>(m1), c, 0xf, 0xf, true));
+        if (kHalf) {
+          const unsigned b2 = static_cast<unsigned>(
+              __builtin_amdgcn_update_dpp(0, static_cast<int>(m2), c, 0xf, 0xf, true));
+          m2 = min(min(m2, b2), max(m1, b1));
+        }
+        m1 = min(m1, b1);
+      };
+      pair_step(std::integral_constant<int, 0xB1>{});
+      pair_step(std::integral_constant<int, 0x4E>{});
+      pair_step(std::integral_constant<int, 0x141>{});
+      pair_step(std::integral_constant<int, 0x140>{});
+      e2n = -1;
+      if (m1 == ~0u || beyond(m1)) return -1;
+      take(m1);
+      if (kHalf && m2 != ~0u && !beyond(m2)) {
+        take(m2);
+        e2n = static_cast<int>(m2 & 63u);
+      }
+      return static_cast<int>(m1 & 63u);
+    };
+#elif LC_ROW_SELECT
     // nearest first: the entry of the row's smallest key, taken out of the
     // list; -1 once the list is empty or its nearest bound is beyond the
     // row's threshold (every later entry is then beyond it too)
@@ -2109,7 +2229,12 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
 #endif
     int i = 0;
     int e = next(i);
-    fetch(e);
+#if LC_ROW_SELECT && LC_KEYS_REG
+    int e2 = e2n;
+#else
+    int e2 = (kHalf && e >= 0) ? next(i) : -1;
+#endif
+    fetch(e, e2);
 #pragma unroll 1
     while (__ballot(e >= 0)) {
       double shx, shy, shz;
@@ -2126,7 +2251,12 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       const bool swept = e >= 0;
       i += 1;
       e = swept ? next(i) : -1;
-      fetch(e);
+#if LC_ROW_SELECT && LC_KEYS_REG
+      e2 = swept ? e2n : -1;
+#else
+      if (kHalf) e2 = e >= 0 ? next(i) : -1;
+#endif
+      fetch(e, e2);
       if (kStats) st_tiles += 1;
       if (kStats) tile_acc = tile_acc_lo = tile_acc_hi = false;
       sweep(std::false_type{});
@@ -2264,57 +2394,149 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
               lbt = gx * gx + gy * gy + gz * gz;
               need = lbt <= rthr;
             }
-            // exact per-source test of the tiles that passed the row box
-            if (__ballot(need)) {
-              if (kStats) st_exact += 1;
-              const double tq = live ? thr : -inf;
-              bool any = false;
-              double lbs = inf;  // smallest per-source bound (a tighter key)
-              auto probe_src = [&](auto I) LC_INLINE {
-                constexpr int ii = decltype(I)::value;
-                const double qx = row_bcast<ii>(px);
-                const double qy = row_bcast<ii>(py);
-                const double qz = row_bcast<ii>(pz);
-                const double qt = row_bcast<ii>(tq);
-                const double gx = fmax(fmax(bx0 - qx, qx - bx1), 0.0);
-                const double gy = fmax(fmax(by0 - qy, qy - by1), 0.0);
-                const double gz = fmax(fmax(bz0 - qz, qz - bz1), 0.0);
-                const double ls = gx * gx + gy * gy + gz * gz;
-                any |= ls <= qt;
-                // only sources that can still take the tile bound its key
-                if (LC_ROW_SELECT) lbs = key_min(lbs, ls <= qt ? ls : inf);
-              };
-              probe_src(std::integral_constant<int, 0>{});
-              probe_src(std::integral_constant<int, 1>{});
-              probe_src(std::integral_constant<int, 2>{});
-              probe_src(std::integral_constant<int, 3>{});
-              probe_src(std::integral_constant<int, 4>{});
-              probe_src(std::integral_constant<int, 5>{});
-              probe_src(std::integral_constant<int, 6>{});
-              probe_src(std::integral_constant<int, 7>{});
-              probe_src(std::integral_constant<int, 8>{});
-              probe_src(std::integral_constant<int, 9>{});
-              probe_src(std::integral_constant<int, 10>{});
-              probe_src(std::integral_constant<int, 11>{});
-              probe_src(std::integral_constant<int, 12>{});
-              probe_src(std::integral_constant<int, 13>{});
-              probe_src(std::integral_constant<int, 14>{});
-              probe_src(std::integral_constant<int, 15>{});
-              need = need && any;
-              if (LC_ROW_SELECT) lbt = key_max(lbt, lbs);
-            }
-            const unsigned long long bal = __ballot(need);
-            const unsigned rm = static_cast<unsigned>(bal >> rbase) & 0xffffu;
-            if (need) {
-              const int pos = lcount + __popc(rm & ((1u << li) - 1u));
-              const int e = on2 - t * kTile;
-              s_list[wv][rw][pos] = make_int2(oa0 + t * kTile, ocode | ((e < kTile ? e : kTile) << 5));
-              if (LC_ROW_SELECT) {
-                s_key[wv][rw][pos] =
-                    (__float_as_uint(__double2float_rd(lbt)) & ~63u) | static_cast<unsigned>(pos);
+            if constexpr (kHalf) {
+              // exact per-source test of each half of the tiles that passed
+              // the row box, with the half tiles' own boxes
+              bool need_a = false, need_b = false;
+              double lb_a = inf, lb_b = inf;
+              const int e = on2 - t * kTile;  // members of the tile
+              if (__ballot(need)) {
+                if (kStats) st_exact += 1;
+                double ax0 = 0, ax1 = 0, ay0 = 0, ay1 = 0, az0 = 0, az1 = 0;
+                double cx0 = 0, cx1 = 0, cy0 = 0, cy1 = 0, cz0 = 0, cz1 = 0;
+                if (need) {
+                  double shx, shy, shz;
+                  shift_of(ocode, shx, shy, shz);
+                  const Box ha = A.hbox[2 * (ot0 + t)];
+                  const Box hb = A.hbox[2 * (ot0 + t) + 1];
+                  ax0 = ha.lx + shx;
+                  ax1 = ha.hx + shx;
+                  ay0 = ha.ly + shy;
+                  ay1 = ha.hy + shy;
+                  az0 = ha.lz + shz;
+                  az1 = ha.hz + shz;
+                  cx0 = hb.lx + shx;
+                  cx1 = hb.hx + shx;
+                  cy0 = hb.ly + shy;
+                  cy1 = hb.hy + shy;
+                  cz0 = hb.lz + shz;
+                  cz1 = hb.hz + shz;
+                }
+                const double tq = live ? thr : -inf;
+                bool any_a = false, any_b = false;
+                auto probe_src = [&](auto I) LC_INLINE {
+                  constexpr int ii = decltype(I)::value;
+                  const double qx = row_bcast<ii>(px);
+                  const double qy = row_bcast<ii>(py);
+                  const double qz = row_bcast<ii>(pz);
+                  const double qt = row_bcast<ii>(tq);
+                  {
+                    const double gx = fmax(fmax(ax0 - qx, qx - ax1), 0.0);
+                    const double gy = fmax(fmax(ay0 - qy, qy - ay1), 0.0);
+                    const double gz = fmax(fmax(az0 - qz, qz - az1), 0.0);
+                    const double ls = gx * gx + gy * gy + gz * gz;
+                    any_a |= ls <= qt;
+                    lb_a = key_min(lb_a, ls <= qt ? ls : inf);
+                  }
+                  {
+                    const double gx = fmax(fmax(cx0 - qx, qx - cx1), 0.0);
+                    const double gy = fmax(fmax(cy0 - qy, qy - cy1), 0.0);
+                    const double gz = fmax(fmax(cz0 - qz, qz - cz1), 0.0);
+                    const double ls = gx * gx + gy * gy + gz * gz;
+                    any_b |= ls <= qt;
+                    lb_b = key_min(lb_b, ls <= qt ? ls : inf);
+                  }
+                };
+                probe_src(std::integral_constant<int, 0>{});
+                probe_src(std::integral_constant<int, 1>{});
+                probe_src(std::integral_constant<int, 2>{});
+                probe_src(std::integral_constant<int, 3>{});
+                probe_src(std::integral_constant<int, 4>{});
+                probe_src(std::integral_constant<int, 5>{});
+                probe_src(std::integral_constant<int, 6>{});
+                probe_src(std::integral_constant<int, 7>{});
+                probe_src(std::integral_constant<int, 8>{});
+                probe_src(std::integral_constant<int, 9>{});
+                probe_src(std::integral_constant<int, 10>{});
+                probe_src(std::integral_constant<int, 11>{});
+                probe_src(std::integral_constant<int, 12>{});
+                probe_src(std::integral_constant<int, 13>{});
+                probe_src(std::integral_constant<int, 14>{});
+                probe_src(std::integral_constant<int, 15>{});
+                // a tile of 8 or fewer has an empty second half, whose
+                // (+inf, -inf) box passes an infinite threshold
+                need_a = need && any_a;
+                need_b = need && any_b && e > 8;
               }
+              const unsigned rma = static_cast<unsigned>(__ballot(need_a) >> rbase) & 0xffffu;
+              const unsigned rmb = static_cast<unsigned>(__ballot(need_b) >> rbase) & 0xffffu;
+              const unsigned below = (1u << li) - 1u;
+              if (need_a) {
+                const int pos = lcount + __popc(rma & below);
+                s_list[wv][rw][pos] = make_int2(oa0 + t * kTile, ocode | ((e < 8 ? e : 8) << 5));
+                s_key[wv][rw][pos] = (__float_as_uint(__double2float_rd(lb_a)) & ~63u) |
+                                     static_cast<unsigned>(pos);
+              }
+              if (need_b) {
+                const int pos = lcount + __popc(rma) + __popc(rmb & below);
+                s_list[wv][rw][pos] = make_int2(oa0 + t * kTile + 8, ocode | ((e - 8) << 5));
+                s_key[wv][rw][pos] = (__float_as_uint(__double2float_rd(lb_b)) & ~63u) |
+                                     static_cast<unsigned>(pos);
+              }
+              lcount += __popc(rma) + __popc(rmb);
+            } else {
+            // exact per-source test of the tiles that passed the row box
+              if (__ballot(need)) {
+                if (kStats) st_exact += 1;
+                const double tq = live ? thr : -inf;
+                bool any = false;
+                double lbs = inf;  // smallest per-source bound (a tighter key)
+                auto probe_src = [&](auto I) LC_INLINE {
+                  constexpr int ii = decltype(I)::value;
+                  const double qx = row_bcast<ii>(px);
+                  const double qy = row_bcast<ii>(py);
+                  const double qz = row_bcast<ii>(pz);
+                  const double qt = row_bcast<ii>(tq);
+                  const double gx = fmax(fmax(bx0 - qx, qx - bx1), 0.0);
+                  const double gy = fmax(fmax(by0 - qy, qy - by1), 0.0);
+                  const double gz = fmax(fmax(bz0 - qz, qz - bz1), 0.0);
+                  const double ls = gx * gx + gy * gy + gz * gz;
+                  any |= ls <= qt;
+                  // only sources that can still take the tile bound its key
+                  if (LC_ROW_SELECT) lbs = key_min(lbs, ls <= qt ? ls : inf);
+                };
+                probe_src(std::integral_constant<int, 0>{});
+                probe_src(std::integral_constant<int, 1>{});
+                probe_src(std::integral_constant<int, 2>{});
+                probe_src(std::integral_constant<int, 3>{});
+                probe_src(std::integral_constant<int, 4>{});
+                probe_src(std::integral_constant<int, 5>{});
+                probe_src(std::integral_constant<int, 6>{});
+                probe_src(std::integral_constant<int, 7>{});
+                probe_src(std::integral_constant<int, 8>{});
+                probe_src(std::integral_constant<int, 9>{});
+                probe_src(std::integral_constant<int, 10>{});
+                probe_src(std::integral_constant<int, 11>{});
+                probe_src(std::integral_constant<int, 12>{});
+                probe_src(std::integral_constant<int, 13>{});
+                probe_src(std::integral_constant<int, 14>{});
+                probe_src(std::integral_constant<int, 15>{});
+                need = need && any;
+                if (LC_ROW_SELECT) lbt = key_max(lbt, lbs);
+              }
+              const unsigned long long bal = __ballot(need);
+              const unsigned rm = static_cast<unsigned>(bal >> rbase) & 0xffffu;
+              if (need) {
+                const int pos = lcount + __popc(rm & ((1u << li) - 1u));
+                const int e = on2 - t * kTile;
+                s_list[wv][rw][pos] = make_int2(oa0 + t * kTile, ocode | ((e < kTile ? e : kTile) << 5));
+                if (LC_ROW_SELECT) {
+                  s_key[wv][rw][pos] =
+                      (__float_as_uint(__double2float_rd(lbt)) & ~63u) | static_cast<unsigned>(pos);
+                }
+              }
+  lcount += __popc(rm);
             }
-            lcount += __popc(rm);
             p0 += 16;
           }
         }
@@ -3063,6 +3285,13 @@ struct Sizes {
   int kmax_exact = 1;
 };
 
+// Whether the row walk of some frame lists half tiles (a heap of
+// LC_ROW_HALF_KC slots or more: k above half of it), so the preparation
+// also builds the half-tile boxes.
+bool half_boxes(const Sizes& z) {
+  return LC_ROW_HALF != 0 && z.kmax_exact > LC_ROW_HALF_KC / 2;
+}
+
 Sizes sizes_of(int F, long long ntot, long long ctot, int kmax_exact,
                long long heap_budget) {
   Sizes z;
@@ -3131,7 +3360,8 @@ void buffer_bytes(const Sizes& z, size_t (&b)[kNumBufs]) {
       sizeof(Cand) * (N + kPad),                     // cand
       sizeof(int) * N,                               // scell
       sizeof(int) * N,                               // byidx
-      sizeof(Box) * static_cast<size_t>(z.tcap),     // tbox
+      // tbox, then (half tiles) two boxes per tile
+      sizeof(Box) * static_cast<size_t>(z.tcap) * (half_boxes(z) ? 3 : 1),
       sizeof(int) * static_cast<size_t>(z.tcap),     // tile_cell
       sizeof(Box) * C,                               // cbox
       sizeof(int) * N,                               // defer
@@ -3194,7 +3424,8 @@ void launch_prep(const Sizes& z, const Bufs& B, hipStream_t st, hipEvent_t* ev) 
   k_cell_sort<<<item_blocks, 256, 0, st>>>(B.items, B.item_off, ctot, B.off, B.tmp_key,
                                            B.frames, B.fcb, F, B.cand, B.scell, B.byidx);
   k_tile_box<<<item_blocks, 256, 0, st>>>(B.items, B.item_off, ctot, B.off, B.tile_off,
-                                          B.cand, B.tbox);
+                                          B.cand, B.tbox,
+                                          half_boxes(z) ? B.tbox + z.tcap : nullptr);
   k_cell_box<<<blocks_for(ctot, 256), 256, 0, st>>>(B.off, B.tile_off, ctot, B.tbox, B.cbox);
   check(hipGetLastError(), "cell_sort launch");
   if (ev) check(hipEventRecord(ev[4], st), "event record");
@@ -3216,6 +3447,7 @@ WalkArgs walk_args(const Sizes& z, const Bufs& B, int dense_min) {
   wa.item_off = B.item_off;
   wa.ncells = z.ctot;
   wa.tbox = B.tbox;
+  wa.hbox = half_boxes(z) ? B.tbox + z.tcap : nullptr;
   wa.cbox = B.cbox;
   wa.dense_min = dense_min;
   wa.tile_cell = B.tile_cell;
@@ -3237,6 +3469,11 @@ void launch_walk(const Sizes& z, const Bufs& B, const WalkArgs& wa,
   // row walk: one wave per 4 source tiles, frames on wave boundaries
   const unsigned row_blocks = blocks_for(z.tcap / 4 + z.F + 1, 4);
   if (ev) check(hipEventRecord(ev[8], st), "event record");
+  if (LC_ROW_HALF != 0 && row_mode && wa.hbox == nullptr) {
+    for (int kc = LC_ROW_HALF_KC; kc <= 32; ++kc) {
+      if (any_fast[kc]) throw std::runtime_error("row walk: half-tile boxes not built");
+    }
+  }
 #define LC_WALK(KC)                                                             \
   if (any_fast[KC] && row_mode) {                                               \
     if (wa.stats != nullptr) {                                                  \
