@@ -799,6 +799,22 @@ __device__ __forceinline__ void bitonic_sort(long long (&d)[KC], int (&j)[KC]) {
   }
 }
 
+// Ascending sort of 8 slots with Batcher's odd-even merge sort: 19
+// compare-exchanges in 6 layers (the bitonic network takes 24). The buffer
+// merge uses it unless LC_SORT8_OE is 0.
+#ifndef LC_SORT8_OE
+#define LC_SORT8_OE 1
+#endif
+__device__ __forceinline__ void oddeven_sort8(long long (&d)[8], int (&j)[8]) {
+  auto cs = [&](int a, int b) { cswap(d[a], j[a], d[b], j[b], true); };
+  cs(0, 1), cs(2, 3), cs(4, 5), cs(6, 7);
+  cs(0, 2), cs(1, 3), cs(4, 6), cs(5, 7);
+  cs(1, 2), cs(5, 6);
+  cs(0, 4), cs(1, 5), cs(2, 6), cs(3, 7);
+  cs(2, 4), cs(3, 5);
+  cs(1, 2), cs(3, 4), cs(5, 6);
+}
+
 // Merge n buffered candidates (LDS column of this lane, stride kWave) into
 // the top-k: sort the batch, keep the KC smallest of (top, batch) with the
 // bitonic min trick (top ascending against batch descending), clean the
@@ -900,7 +916,7 @@ __device__ __forceinline__ void insert_merge_regs(Top<KC>& h, const long long (&
 }
 
 // Merge up to 8 buffered candidates into the top-k (KC >= 8) with a fixed
-// network: sort the 8 (bitonic), take the element-wise min of the top
+// network: sort the 8 (odd-even network), take the element-wise min of the top
 // (ascending) and the batch reversed and padded with +inf (top[KC - 8 + t]
 // against batch[7 - t]); that min holds the KC smallest of both and is
 // bitonic, so one bitonic clean over all KC slots sorts it. Same kept set
@@ -909,7 +925,11 @@ __device__ __forceinline__ void insert_merge_regs(Top<KC>& h, const long long (&
 template <int KC>
 __device__ __forceinline__ void merge8_regs(Top<KC>& h, long long (&b)[8],
                                             int (&bx)[8], double& min_rej) {
+#if LC_SORT8_OE
+  oddeven_sort8(b, bx);
+#else
   bitonic_sort<8>(b, bx);
+#endif
   // an empty slot or pad is (+inf, kEmptyJ), so the d2 a pair drops feeds
   // min_rej unconditionally: +inf leaves it unchanged
 #pragma unroll
