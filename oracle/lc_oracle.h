@@ -19,12 +19,17 @@
  *                        0 <= s < 1 and primary-cell invariants.
  *   shift / dist2      - lattice_shift(-1,0,0) == [-10,0,0] exactly,
  *                        MIC 0.64 known answers.
- *   walk + heap        - independent 27-image brute force (numpy) and
- *                        the restated knearest_brute on every
- *                        reference known-answer and LCG-seeded case
+ *   walk + heap        - independent 27-image brute force (numpy,
+ *                        bitwise d2) and the restated knearest_brute on
+ *                        every reference known-answer and LCG-seeded case
  *                        (tests/agree.rs:30-558, tests/abi.rs:83-367).
- *   exact d2 bits and boundary-tie retention are pinned by
- *   construction only (no Rust toolchain here to import the reference).
+ *   boundary ties      - hand-derived answers for exact d2 ties across
+ *                        cells, inside a cell, across evictions and
+ *                        shells (tests/golden/tie_answers.json), which
+ *                        builds with a planted mistake (LC_ORACLE_PLANT)
+ *                        fail.
+ * Host sanitizers: tests/sanitize/oracle_check.c runs it under ASan,
+ * UBSan and TSan.
  */
 #ifndef LC_ORACLE_H
 #define LC_ORACLE_H
