@@ -7,15 +7,22 @@
 //               cell); the 9-bit Hilbert index of the point's 8x8x8
 //               sub-cell rides along
 //   scan x2     single-pass decoupled look-back exclusive scans over cell
-//               counts; one packed 64-bit status word per 2048-cell tile
-//               carries the occupant prefix and a second prefix (16-point
-//               tiles, then 64-source chunks)
+//               counts (DPP row shifts and row broadcasts within the wave);
+//               one packed 64-bit status word per 2048-cell tile carries the
+//               occupant prefix and a second prefix (16-point tiles, then
+//               64-source chunks)
 //   scatter     arrival-order scatter of (key, id) into cell segments
 //   cell_sort   one wave per 64-member chunk: rank by (sub-cell key, id) for
 //               the walk layout and by id for the exact-order replay,
-//               re-fold, write SoA FP64 coordinates
-//   boxes       exact bounding boxes of every 16-point tile and every cell
+//               re-fold, write 32-byte candidate records (x, y, z, id)
+//   boxes       exact bounding boxes of every 16-point tile (and of its two
+//               8-point halves) and every cell
 // Walks:
+//   walk_rows   the default: one 16-lane row per 16-source tile, four rows
+//               per wave; candidate tiles listed per shell with exact box
+//               tests, drained nearest first, candidates broadcast within
+//               the row through DPP, accepted ones buffered in LDS and
+//               merged into per-lane register heaps
 //   walk_dense  one wave per 64 sources of one dense home cell; candidates
 //               arrive through scalar loads (uniform across the wave), tiles
 //               whose exact lower-bound distance exceeds every lane's k-th
@@ -393,6 +400,32 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(
   return v;
 }
 
+// v from the lane DPP control CTRL names (0 where it names none, or outside
+// the rows ROWS selects), both 32-bit halves moved by DPP
+template <int CTRL, int ROWS>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(v >> 32), CTRL, ROWS, 0xf, false);
+  return (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+         static_cast<unsigned long long>(static_cast<unsigned>(lo));
+}
+
+// Inclusive scan over the 64 lanes of a wave through DPP: row_shr by 1, 2,
+// 4 and 8 scans each 16-lane row (lanes shifted in from outside the row
+// read 0), row_bcast:15 adds row 0's total to row 1 and row 2's to row 3,
+// row_bcast:31 adds the total of rows 0-1 to rows 2 and 3. Every lane must
+// be active.
+__device__ __forceinline__ unsigned long long wave_incl_scan_u64(
+    unsigned long long v) {
+  v += dpp_u64<0x111, 0xf>(v);
+  v += dpp_u64<0x112, 0xf>(v);
+  v += dpp_u64<0x114, 0xf>(v);
+  v += dpp_u64<0x118, 0xf>(v);
+  v += dpp_u64<0x142, 0xa>(v);
+  v += dpp_u64<0x143, 0xc>(v);
+  return v;
+}
+
 // Single-pass exclusive scan with decoupled look-back. Tiles take ids
 // from an atomic counter, so a tile only ever waits on tiles that are
 // already resident. Each tile publishes its aggregate at once and its
@@ -422,12 +455,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(
     ex[t] = sum;
     sum += pack_cell(c, sh);
   }
-  unsigned long long incl = sum;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const unsigned long long up = __shfl_up(incl, d, kWave);
-    if (lane >= d) incl += up;
-  }
+  const unsigned long long incl = wave_incl_scan_u64(sum);
   if (lane == kWave - 1) s_wave[wave] = incl;
   __syncthreads();
   unsigned long long wave_ex = 0, tile_total = 0;
