@@ -72,20 +72,24 @@ int lc_gpu_graph_capture(lc_gpu_workspace *ws, const lc_gpu_frame *frames,
 int lc_gpu_graph_launch(lc_gpu_workspace *ws, int wait);
 int lc_gpu_graph_reset(lc_gpu_workspace *ws);
 
-/* Time the batch under each walk split and keep the fastest for batches of
- * its size bucket (largest k bucket, log2 point count, log2 mean cell
- * occupancy). Results do not depend on the split. */
+/* Time the batch under each walk split (the row walk launched per group
+ * or persistent, and the wave-per-cell splits) and keep the fastest for
+ * batches of its size bucket (largest k bucket, log2 point count, log2
+ * mean cell occupancy). Results do not depend on the split. */
 int lc_gpu_autotune(lc_gpu_workspace *ws, const lc_gpu_frame *frames,
                     size_t n_frames, int *best_dense_min, double *best_ms);
 
 /* Walk split. LC_GPU_ROW_WALK (-1): one 16-lane row per 16-source tile
- * of a home cell. A positive value: home cells with at least that many
- * sources take the wave-per-cell walk, the rest the lane-per-source walk.
- * 0 (the default) uses the autotuned split for the batch's size bucket,
- * else the row walk. LINKCELL_DENSE_MIN sets a positive split at
- * workspace creation; LINKCELL_TPP=1, 16 or 64 selects the all-lane, row
- * or all-wave walk. */
+ * of a home cell. LC_GPU_ROW_WALK_PERSISTENT (-2): the same walk as one
+ * persistent launch per heap bucket, its grid as large as the GPU holds at
+ * once and its waves taking groups of four tiles from a device counter.
+ * A positive value: home cells with at least that many sources take the
+ * wave-per-cell walk, the rest the lane-per-source walk. 0 (the default)
+ * uses the autotuned split for the batch's size bucket, else the row walk.
+ * LINKCELL_DENSE_MIN sets a positive split at workspace creation;
+ * LINKCELL_TPP=1, 16 or 64 selects the all-lane, row or all-wave walk. */
 #define LC_GPU_ROW_WALK (-1)
+#define LC_GPU_ROW_WALK_PERSISTENT (-2)
 int lc_gpu_set_dense_min(lc_gpu_workspace *ws, int dense_min);
 
 /* Walk schedule. Results are identical under every schedule.

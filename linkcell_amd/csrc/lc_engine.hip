@@ -77,6 +77,7 @@ constexpr int kChunkShift = 6;  // sources per dense work item (one wave)
 constexpr int kPad = 16;        // slack elements after SoA arrays (group reads)
 constexpr int kNeverDense = 1 << 30;  // dense_min that keeps every cell sparse
 constexpr int kRowWalk = -1;          // split value selecting the row walk
+constexpr int kRowWalkPersist = -2;   // the row walk as one persistent launch per heap bucket
 constexpr int kExactWaveMaxK = 2 * kWave;  // wave-per-source exact walk: k up to two slots per lane
 
 // Cell-ordered candidate record: folded coordinates and the point's index
@@ -1164,6 +1165,10 @@ struct WalkArgs {
   // packs the tiles of every 16 by that depth
   int depth;
   const int* tile_perm;
+  // persistent row walk (null: one wave per group of four tiles): waves
+  // take groups from this counter until n_waves groups are taken
+  int* queue;
+  int n_waves;
 };
 
 enum {
@@ -1834,8 +1839,11 @@ struct RowOcc {
 #define LC_ROW_BLOCKS(KC) RowOcc<KC>::kMinBlocks
 #endif
 // kDepth: the depth schedule (rows list shells 1 .. predicted depth in one
-// pass); without it every row of a wave is in the same shell.
-template <int KC, bool kStats, bool kDepth>
+// pass); without it every row of a wave is in the same shell. kPersist: the
+// grid fills the GPU once and every wave takes groups of four tiles from a
+// device counter until none is left (one persistent launch per heap
+// bucket); without it each wave walks the group its position names.
+template <int KC, bool kStats, bool kDepth, bool kPersist>
 __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A) {
   constexpr int W = 4;   // waves per block
   constexpr int BC = 8;  // buffered candidates per lane between merges
@@ -1869,835 +1877,847 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   const int li = lane & 15;
   const int rw = lane >> 4;
   const int rbase = lane & 48;
-  const int wg = uni(static_cast<int>(blockIdx.x) * W + wv);
-  // the frame whose waves contain wg
-  int f = 0;
-  {
-    int lo = 0, hi = A.F - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (row_wave0(A.tile_off, A.fcb, mid) <= wg) lo = mid;
-      else hi = mid - 1;
-    }
-    f = uni(lo);
-  }
-  const FrameDesc& d = A.fr[f];
-  if (d.kc != KC) return;
-  const int tb = A.tile_off[A.fcb[f]];
-  const int nt_frame = A.tile_off[A.fcb[f + 1]] - tb;
-  const int tf = (wg - row_wave0(A.tile_off, A.fcb, f)) * 4 + rw;
-  if (__ballot(tf < nt_frame) == 0) return;
-  const bool has_tile = tf < nt_frame;
-  int T = tb + (has_tile ? tf : 0);
-  if (A.tile_perm != nullptr && has_tile) T = A.tile_perm[T];
-  const int g = A.tile_cell[T];
-  const int o0 = A.off[g];
-  const int cnt = A.off[g + 1] - o0;
-  const int tl = T - A.tile_off[g];
-  const int srel = tl * kTile + li;
-  bool valid = has_tile && srel < cnt;
-  const int s = o0 + (valid ? srel : 0);
-  const bool frame_ok = (A.fflags[f] & kFrameNonFinite) == 0 && d.r_safe >= 1;
-  if (!frame_ok) {
-    if (valid) defer_source(s, A.defer_list, A.defer_count);
-    return;
-  }
-  const int k = d.k;
-  const int nx = d.nx, ny = d.ny, nz = d.nz;
-  const int cb = static_cast<int>(d.cell_base);
-  const int c = g - cb;
-  const int hx = c % nx;
-  const int hy = (c / nx) % ny;
-  const int hz = c / (nx * ny);
-  const Cand mc = A.cand[s];
-  const double px = mc.x, py = mc.y, pz = mc.z;
-  const int me = static_cast<int>(mc.id);
-  const double inf = as_f64(kInfBits);
-  // bounding box of the row's sources, kept in LDS (read only while
-  // listing tiles)
-  {
-    const double v0 = row_min16(valid ? px : inf);
-    const double v1 = row_max16(valid ? px : -inf);
-    const double v2 = row_min16(valid ? py : inf);
-    const double v3 = row_max16(valid ? py : -inf);
-    const double v4 = row_min16(valid ? pz : inf);
-    const double v5 = row_max16(valid ? pz : -inf);
-    if (li == 0) {
-      double* hb = s_hull[wv][rw];
-      hb[0] = v0;
-      hb[1] = v1;
-      hb[2] = v2;
-      hb[3] = v3;
-      hb[4] = v4;
-      hb[5] = v5;
-    }
-  }
-  const double* hull = s_hull[wv][rw];
-
-  // lattice shift of a stencil cell from its wrap code (na+1)*9+(nb+1)*3+nc+1
-  // The 27 lattice shifts of the frame, from the wrap code
-  // (na+1)*9+(nb+1)*3+nc+1: computed once per wave into LDS (lane = code)
-  // by the expressions of Cell::lattice_shift (src/cell.rs:227-247), so a
-  // lookup gives the same bits without re-reading the frame's cell.
-  {
-    if (lane < 27) {
-      const int code = lane;
-      const int na = code / 9 - 1, nb = (code / 3) % 3 - 1, nc = code % 3 - 1;
-      double x, y, z;
-      if (code == 13) {
-        x = y = z = 0.0;
-      } else if (d.ortho) {
-        x = static_cast<double>(na) * d.widths[0];
-        y = static_cast<double>(nb) * d.widths[1];
-        z = static_cast<double>(nc) * d.widths[2];
-      } else {
-        const double fa = na, fb = nb, fc = nc;
-        x = fa * d.h[0] + fb * d.h[3] + fc * d.h[6];
-        y = fa * d.h[1] + fb * d.h[4] + fc * d.h[7];
-        z = fa * d.h[2] + fb * d.h[5] + fc * d.h[8];
-      }
-      s_shift[wv][code][0] = x;
-      s_shift[wv][code][1] = y;
-      s_shift[wv][code][2] = z;
-    }
-    __builtin_amdgcn_wave_barrier();
-  }
-  auto shift_of = [&](int code, double& x, double& y, double& z) LC_INLINE {
-    x = s_shift[wv][code][0];
-    y = s_shift[wv][code][1];
-    z = s_shift[wv][code][2];
-  };
-
-  Top<KC> h;
-  top_init<KC>(h, k);
-  double min_rej = inf;
-  // threshold cap from the predicted k-th distance of the home cell (+inf
-  // without a prediction). Candidates beyond it are never kept; the shell-
-  // end check below proves the kept set exact or defers the source.
-  double thr_cap = inf;
-  if (A.pred != nullptr) {
-    const double q = static_cast<double>(A.pred[g]);
-    thr_cap = q * q;
-  }
-  if (A.pred_tile != nullptr && T < A.pred_tile_n) {
-    const double q = static_cast<double>(A.pred_tile[T]);
-    thr_cap = fmin(thr_cap, q * q);
-  }
-  if (A.pred_src != nullptr && s < A.pred_src_n) {
-    const double q = static_cast<double>(A.pred_src[s]);
-    thr_cap = fmin(thr_cap, q * q);
-  }
-  // predicted depth of the row (1 without a depth schedule, or where the
-  // per-source reach is asked for: the reach is the reference's stop
-  // shell, which only a shell-by-shell walk observes)
-  int rp = 1;
-  if (kDepth && A.pred_tile != nullptr && T < A.pred_tile_n && d.out_reach == nullptr) {
-    const double q = ceil(static_cast<double>(A.pred_tile[T]) / d.cell_min);
-    rp = q >= static_cast<double>(d.r_safe) ? d.r_safe : (q > 1.0 ? static_cast<int>(q) : 1);
-  }
-  bool live = valid;
-  double thr = live ? thr_cap : -inf;  // -inf: finished (never accepts)
-  bool cap_fail = false;
-  double rthr = row_max16(live ? thr : -inf);  // -inf: no live source
-  int nbuf = 0;
-  int done_reach = 0;
-  unsigned st_tests = 0, st_tiles = 0, st_merges = 0, st_rounds = 0,
-           st_cells = 0, st_exact = 0, st_useful = 0;
-  unsigned long long cyc_fill = 0, cyc_drain = 0, cyc_merge = 0;
-  unsigned st_flush = 0;
-  unsigned st_rounds2 = 0, st_useful2 = 0, st_wave2 = 0;
-  unsigned st_dead = 0, st_acc = 0, st_deadh = 0;
-  bool tile_acc = false, tile_acc_lo = false, tile_acc_hi = false;
-  unsigned long long cyc_r2 = 0, cyc_open = 0, cyc_init = 0;
-  int cur_r = 1;
-  const unsigned long long cyc0 = kStats ? __builtin_readcyclecounter() : 0;
-
-  auto flush = [&]() LC_INLINE {
-    const unsigned long long t0 = kStats ? __builtin_readcyclecounter() : 0;
-    if (kStats) st_flush += 1;
-    if (nbuf > 0) {
-      if (kStats) st_merges += 1;
-      if constexpr (KC >= 8) {
-        // the buffer is read straight from the shared arrays (LDS loads)
-        long long b[8];
-        int bx[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          const bool v = t < nbuf;
-          const int4 e = s_buf[wv][t][lane];
-          b[t] = v ? (static_cast<long long>(e.y) << 32) |
-                         static_cast<long long>(static_cast<unsigned>(e.x))
-                   : kInfBits;
-          bx[t] = v ? e.z : kEmptyJ;
-        }
-        merge8_regs<KC>(h, b, bx, min_rej);
-      } else {
-        long long b[8];
-        int bx[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          const int4 e = s_buf[wv][t][lane];
-          b[t] = (static_cast<long long>(e.y) << 32) |
-                 static_cast<long long>(static_cast<unsigned>(e.x));
-          bx[t] = e.z;
-        }
-        insert_merge_regs<KC>(h, b, bx, nbuf, min_rej);
-      }
-      nbuf = 0;
-      thr = live ? fmin(as_f64(h.d[KC - 1]), thr_cap) : -inf;
-    }
-    rthr = row_max16(live ? thr : -inf);
-    if (kStats) cyc_merge += __builtin_readcyclecounter() - t0;
-  };
-
-  double cx = 0.0, cy = 0.0, cz = 0.0;
-  int cj = -1;
-  // One candidate (lane I of each row) against every source of the row.
-  // Empty candidate slots carry NaN coordinates and finished sources a
-  // threshold of -inf: neither is ever accepted. The index is broadcast only
-  // when some lane of the wave accepts (outside the per-lane branch, where
-  // every lane is active for the DPP read). SELF: the candidates are the
-  // row's own sources, and the source itself is skipped; no other tile of
-  // the walk holds it (its own tile is swept only in the opening).
-  auto step = [&](auto I, auto selftag) LC_INLINE {
-    constexpr int ii = decltype(I)::value;
-    constexpr bool SELF = decltype(selftag)::value;
-    const double qx = row_bcast<ii>(cx);
-    const double qy = row_bcast<ii>(cy);
-    const double qz = row_bcast<ii>(cz);
-    const double ddx = qx - px, ddy = qy - py, ddz = qz - pz;
-    const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-    bool acc = d2 <= thr;
-    int qj;
-    if constexpr (kStepBranch) {
-      // the index is broadcast only when some lane of the wave accepts.
-      // SELF reads it before the test, with every lane active: a DPP read
-      // from a lane that is switched off returns 0
-      if (SELF) {
-        const int sj = row_bcast<ii>(cj);
-        acc = acc && sj != me;
-      }
-      if (!__ballot(acc)) return;
-      qj = row_bcast<ii>(cj);
-    } else {
-      // no wave-wide test: the 32-slot heap's rows accept on most steps
-      qj = row_bcast<ii>(cj);
-      if (SELF) acc = acc && qj != me;
-    }
-    if (acc) {
-      // indexed on the shared arrays themselves, so the stores stay LDS
-      // stores whatever the register allocator does with pointers
-      const long long db = __double_as_longlong(d2);
-      s_buf[wv][nbuf][lane] = make_int4(static_cast<int>(db), static_cast<int>(db >> 32), qj, 0);
-      nbuf += 1;
-      if (kStats) {
-        st_acc += 1;
-        tile_acc = true;
-        if (ii < 8) tile_acc_lo = true;
-        else tile_acc_hi = true;
-      }
-    }
-  };
-  auto pair = [&](auto B, auto selftag) LC_INLINE {
-    constexpr int b0 = decltype(B)::value;
-    step(std::integral_constant<int, b0>{}, selftag);
-    step(std::integral_constant<int, b0 + 1>{}, selftag);
-  };
-  // the 16 candidates held in cx..cj (one per lane of each row) against
-  // every source of the row; one merge site, checked after every two
-  // candidates, so buffers fill to BC - 1 or BC before a merge
-  // The switch is entered at pair pr and falls through the following
-  // pairs until a buffer is nearly full; only then does control leave it
-  // for the (single) merge site and come back at the next pair.
-  auto sweep = [&](auto selftag) LC_INLINE {
-    if (kStats) st_rounds += 1;
-    int pr = 0;
 #pragma unroll 1
-    while (pr < 8) {
-      switch (pr) {
-#define LC_PAIR(P)                                              \
-  case P:                                                       \
-    pair(std::integral_constant<int, 2 * (P)>{}, selftag);      \
-    pr = (P) + 1;                                               \
-    if (__ballot(nbuf > BC - 2)) break;                         \
-    [[fallthrough]];
-        LC_PAIR(0)
-        LC_PAIR(1)
-        LC_PAIR(2)
-        LC_PAIR(3)
-        LC_PAIR(4)
-        LC_PAIR(5)
-        LC_PAIR(6)
-        LC_PAIR(7)
-#undef LC_PAIR
-        default: break;
-      }
-      if (__ballot(nbuf > BC - 2)) flush();
+  do {
+    int wgq = 0;
+    if constexpr (kPersist) {
+      int t = 0;
+      if (lane == 0) t = atomicAdd(A.queue, 1);
+      wgq = uni(t);
+      if (wgq >= A.n_waves) break;
+    } else {
+      wgq = uni(static_cast<int>(blockIdx.x) * W + wv);
     }
-  };
+    const int wg = wgq;
+    // the frame whose waves contain wg
+    int f = 0;
+    {
+      int lo = 0, hi = A.F - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (row_wave0(A.tile_off, A.fcb, mid) <= wg) lo = mid;
+        else hi = mid - 1;
+      }
+      f = uni(lo);
+    }
+    const FrameDesc& d = A.fr[f];
+    if (d.kc != KC) continue;
+    const int tb = A.tile_off[A.fcb[f]];
+    const int nt_frame = A.tile_off[A.fcb[f + 1]] - tb;
+    const int tf = (wg - row_wave0(A.tile_off, A.fcb, f)) * 4 + rw;
+    if (__ballot(tf < nt_frame) == 0) continue;
+    const bool has_tile = tf < nt_frame;
+    int T = tb + (has_tile ? tf : 0);
+    if (A.tile_perm != nullptr && has_tile) T = A.tile_perm[T];
+    const int g = A.tile_cell[T];
+    const int o0 = A.off[g];
+    const int cnt = A.off[g + 1] - o0;
+    const int tl = T - A.tile_off[g];
+    const int srel = tl * kTile + li;
+    bool valid = has_tile && srel < cnt;
+    const int s = o0 + (valid ? srel : 0);
+    const bool frame_ok = (A.fflags[f] & kFrameNonFinite) == 0 && d.r_safe >= 1;
+    if (!frame_ok) {
+      if (valid) defer_source(s, A.defer_list, A.defer_count);
+      continue;
+    }
+    const int k = d.k;
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const int cb = static_cast<int>(d.cell_base);
+    const int c = g - cb;
+    const int hx = c % nx;
+    const int hy = (c / nx) % ny;
+    const int hz = c / (nx * ny);
+    const Cand mc = A.cand[s];
+    const double px = mc.x, py = mc.y, pz = mc.z;
+    const int me = static_cast<int>(mc.id);
+    const double inf = as_f64(kInfBits);
+    // bounding box of the row's sources, kept in LDS (read only while
+    // listing tiles)
+    {
+      const double v0 = row_min16(valid ? px : inf);
+      const double v1 = row_max16(valid ? px : -inf);
+      const double v2 = row_min16(valid ? py : inf);
+      const double v3 = row_max16(valid ? py : -inf);
+      const double v4 = row_min16(valid ? pz : inf);
+      const double v5 = row_max16(valid ? pz : -inf);
+      if (li == 0) {
+        double* hb = s_hull[wv][rw];
+        hb[0] = v0;
+        hb[1] = v1;
+        hb[2] = v2;
+        hb[3] = v3;
+        hb[4] = v4;
+        hb[5] = v5;
+      }
+    }
+    const double* hull = s_hull[wv][rw];
 
-  const int nt_home = (cnt + kTile - 1) >> kTileShift;
-  const int tl_nb = tl + 1 < nt_home ? tl + 1 : tl - 1;  // -1: none
-  // a third opening tile, the nearest other neighbour of the row's own
-  // tile in sub-cell order (tl - 1 when tl + 1 opened, else tl - 2): FCC
-  // 8M frame (k = 32) walk 9.73 -> 9.47 ms; clustered 4M frame unchanged
-  // (4.040 against 4.042 ms)
+    // lattice shift of a stencil cell from its wrap code (na+1)*9+(nb+1)*3+nc+1
+    // The 27 lattice shifts of the frame, from the wrap code
+    // (na+1)*9+(nb+1)*3+nc+1: computed once per wave into LDS (lane = code)
+    // by the expressions of Cell::lattice_shift (src/cell.rs:227-247), so a
+    // lookup gives the same bits without re-reading the frame's cell.
+    {
+      if (lane < 27) {
+        const int code = lane;
+        const int na = code / 9 - 1, nb = (code / 3) % 3 - 1, nc = code % 3 - 1;
+        double x, y, z;
+        if (code == 13) {
+          x = y = z = 0.0;
+        } else if (d.ortho) {
+          x = static_cast<double>(na) * d.widths[0];
+          y = static_cast<double>(nb) * d.widths[1];
+          z = static_cast<double>(nc) * d.widths[2];
+        } else {
+          const double fa = na, fb = nb, fc = nc;
+          x = fa * d.h[0] + fb * d.h[3] + fc * d.h[6];
+          y = fa * d.h[1] + fb * d.h[4] + fc * d.h[7];
+          z = fa * d.h[2] + fb * d.h[5] + fc * d.h[8];
+        }
+        s_shift[wv][code][0] = x;
+        s_shift[wv][code][1] = y;
+        s_shift[wv][code][2] = z;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    auto shift_of = [&](int code, double& x, double& y, double& z) LC_INLINE {
+      x = s_shift[wv][code][0];
+      y = s_shift[wv][code][1];
+      z = s_shift[wv][code][2];
+    };
+
+    Top<KC> h;
+    top_init<KC>(h, k);
+    double min_rej = inf;
+    // threshold cap from the predicted k-th distance of the home cell (+inf
+    // without a prediction). Candidates beyond it are never kept; the shell-
+    // end check below proves the kept set exact or defers the source.
+    double thr_cap = inf;
+    if (A.pred != nullptr) {
+      const double q = static_cast<double>(A.pred[g]);
+      thr_cap = q * q;
+    }
+    if (A.pred_tile != nullptr && T < A.pred_tile_n) {
+      const double q = static_cast<double>(A.pred_tile[T]);
+      thr_cap = fmin(thr_cap, q * q);
+    }
+    if (A.pred_src != nullptr && s < A.pred_src_n) {
+      const double q = static_cast<double>(A.pred_src[s]);
+      thr_cap = fmin(thr_cap, q * q);
+    }
+    // predicted depth of the row (1 without a depth schedule, or where the
+    // per-source reach is asked for: the reach is the reference's stop
+    // shell, which only a shell-by-shell walk observes)
+    int rp = 1;
+    if (kDepth && A.pred_tile != nullptr && T < A.pred_tile_n && d.out_reach == nullptr) {
+      const double q = ceil(static_cast<double>(A.pred_tile[T]) / d.cell_min);
+      rp = q >= static_cast<double>(d.r_safe) ? d.r_safe : (q > 1.0 ? static_cast<int>(q) : 1);
+    }
+    bool live = valid;
+    double thr = live ? thr_cap : -inf;  // -inf: finished (never accepts)
+    bool cap_fail = false;
+    double rthr = row_max16(live ? thr : -inf);  // -inf: no live source
+    int nbuf = 0;
+    int done_reach = 0;
+    unsigned st_tests = 0, st_tiles = 0, st_merges = 0, st_rounds = 0,
+             st_cells = 0, st_exact = 0, st_useful = 0;
+    unsigned long long cyc_fill = 0, cyc_drain = 0, cyc_merge = 0;
+    unsigned st_flush = 0;
+    unsigned st_rounds2 = 0, st_useful2 = 0, st_wave2 = 0;
+    unsigned st_dead = 0, st_acc = 0, st_deadh = 0;
+    bool tile_acc = false, tile_acc_lo = false, tile_acc_hi = false;
+    unsigned long long cyc_r2 = 0, cyc_open = 0, cyc_init = 0;
+    int cur_r = 1;
+    const unsigned long long cyc0 = kStats ? __builtin_readcyclecounter() : 0;
+
+    auto flush = [&]() LC_INLINE {
+      const unsigned long long t0 = kStats ? __builtin_readcyclecounter() : 0;
+      if (kStats) st_flush += 1;
+      if (nbuf > 0) {
+        if (kStats) st_merges += 1;
+        if constexpr (KC >= 8) {
+          // the buffer is read straight from the shared arrays (LDS loads)
+          long long b[8];
+          int bx[8];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) {
+            const bool v = t < nbuf;
+            const int4 e = s_buf[wv][t][lane];
+            b[t] = v ? (static_cast<long long>(e.y) << 32) |
+                           static_cast<long long>(static_cast<unsigned>(e.x))
+                     : kInfBits;
+            bx[t] = v ? e.z : kEmptyJ;
+          }
+          merge8_regs<KC>(h, b, bx, min_rej);
+        } else {
+          long long b[8];
+          int bx[8];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) {
+            const int4 e = s_buf[wv][t][lane];
+            b[t] = (static_cast<long long>(e.y) << 32) |
+                   static_cast<long long>(static_cast<unsigned>(e.x));
+            bx[t] = e.z;
+          }
+          insert_merge_regs<KC>(h, b, bx, nbuf, min_rej);
+        }
+        nbuf = 0;
+        thr = live ? fmin(as_f64(h.d[KC - 1]), thr_cap) : -inf;
+      }
+      rthr = row_max16(live ? thr : -inf);
+      if (kStats) cyc_merge += __builtin_readcyclecounter() - t0;
+    };
+
+    double cx = 0.0, cy = 0.0, cz = 0.0;
+    int cj = -1;
+    // One candidate (lane I of each row) against every source of the row.
+    // Empty candidate slots carry NaN coordinates and finished sources a
+    // threshold of -inf: neither is ever accepted. The index is broadcast only
+    // when some lane of the wave accepts (outside the per-lane branch, where
+    // every lane is active for the DPP read). SELF: the candidates are the
+    // row's own sources, and the source itself is skipped; no other tile of
+    // the walk holds it (its own tile is swept only in the opening).
+    auto step = [&](auto I, auto selftag) LC_INLINE {
+      constexpr int ii = decltype(I)::value;
+      constexpr bool SELF = decltype(selftag)::value;
+      const double qx = row_bcast<ii>(cx);
+      const double qy = row_bcast<ii>(cy);
+      const double qz = row_bcast<ii>(cz);
+      const double ddx = qx - px, ddy = qy - py, ddz = qz - pz;
+      const double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+      bool acc = d2 <= thr;
+      int qj;
+      if constexpr (kStepBranch) {
+        // the index is broadcast only when some lane of the wave accepts.
+        // SELF reads it before the test, with every lane active: a DPP read
+        // from a lane that is switched off returns 0
+        if (SELF) {
+          const int sj = row_bcast<ii>(cj);
+          acc = acc && sj != me;
+        }
+        if (!__ballot(acc)) return;
+        qj = row_bcast<ii>(cj);
+      } else {
+        // no wave-wide test: the 32-slot heap's rows accept on most steps
+        qj = row_bcast<ii>(cj);
+        if (SELF) acc = acc && qj != me;
+      }
+      if (acc) {
+        // indexed on the shared arrays themselves, so the stores stay LDS
+        // stores whatever the register allocator does with pointers
+        const long long db = __double_as_longlong(d2);
+        s_buf[wv][nbuf][lane] = make_int4(static_cast<int>(db), static_cast<int>(db >> 32), qj, 0);
+        nbuf += 1;
+        if (kStats) {
+          st_acc += 1;
+          tile_acc = true;
+          if (ii < 8) tile_acc_lo = true;
+          else tile_acc_hi = true;
+        }
+      }
+    };
+    auto pair = [&](auto B, auto selftag) LC_INLINE {
+      constexpr int b0 = decltype(B)::value;
+      step(std::integral_constant<int, b0>{}, selftag);
+      step(std::integral_constant<int, b0 + 1>{}, selftag);
+    };
+    // the 16 candidates held in cx..cj (one per lane of each row) against
+    // every source of the row; one merge site, checked after every two
+    // candidates, so buffers fill to BC - 1 or BC before a merge
+    // The switch is entered at pair pr and falls through the following
+    // pairs until a buffer is nearly full; only then does control leave it
+    // for the (single) merge site and come back at the next pair.
+    auto sweep = [&](auto selftag) LC_INLINE {
+      if (kStats) st_rounds += 1;
+      int pr = 0;
+#pragma unroll 1
+      while (pr < 8) {
+        switch (pr) {
+#define LC_PAIR(P)                                              \
+    case P:                                                       \
+      pair(std::integral_constant<int, 2 * (P)>{}, selftag);      \
+      pr = (P) + 1;                                               \
+      if (__ballot(nbuf > BC - 2)) break;                         \
+      [[fallthrough]];
+          LC_PAIR(0)
+          LC_PAIR(1)
+          LC_PAIR(2)
+          LC_PAIR(3)
+          LC_PAIR(4)
+          LC_PAIR(5)
+          LC_PAIR(6)
+          LC_PAIR(7)
+#undef LC_PAIR
+          default: break;
+        }
+        if (__ballot(nbuf > BC - 2)) flush();
+      }
+    };
+
+    const int nt_home = (cnt + kTile - 1) >> kTileShift;
+    const int tl_nb = tl + 1 < nt_home ? tl + 1 : tl - 1;  // -1: none
+    // a third opening tile, the nearest other neighbour of the row's own
+    // tile in sub-cell order (tl - 1 when tl + 1 opened, else tl - 2): FCC
+    // 8M frame (k = 32) walk 9.73 -> 9.47 ms; clustered 4M frame unchanged
+    // (4.040 against 4.042 ms)
 #ifndef LC_OPEN3
 #define LC_OPEN3 1
 #endif
-  const int tl_nb2 = !LC_OPEN3 ? -1
-                     : (tl_nb == tl + 1 ? (tl >= 1 ? tl - 1 : (tl + 2 < nt_home ? tl + 2 : -1))
-                                        : (tl >= 2 ? tl - 2 : -1));
-  const double nan = __builtin_nan("");
-  // Walk, at most twice: a row some of whose sources the cap may have kept
-  // from the reference's stop (heap not full at a shell end with the cap
-  // below the stop bound) is walked again, whole and uncapped, by the same
-  // wave once every other row has finished.
+    const int tl_nb2 = !LC_OPEN3 ? -1
+                       : (tl_nb == tl + 1 ? (tl >= 1 ? tl - 1 : (tl + 2 < nt_home ? tl + 2 : -1))
+                                          : (tl >= 2 ? tl - 2 : -1));
+    const double nan = __builtin_nan("");
+    // Walk, at most twice: a row some of whose sources the cap may have kept
+    // from the reference's stop (heap not full at a shell end with the cap
+    // below the stop bound) is walked again, whole and uncapped, by the same
+    // wave once every other row has finished.
 #pragma unroll 1
-  for (int attempt = 0; attempt < 2; ++attempt) {
-  // opening: the row's own tile (its sources are the candidates, no
-  // shift), then the next tiles of the home cell in sub-cell order
-  const unsigned long long to0 = kStats ? __builtin_readcyclecounter() : 0;
-  if (kStats && attempt == 0) cyc_init = to0 - cyc0;
-  // the next tile's candidates are loaded before the own tile is swept, so
-  // the load overlaps that sweep
-  double nbx = nan, nby = nan, nbz = nan;
-  int nbj = -1;
-  if (has_tile && tl_nb >= 0 && tl_nb * kTile + li < cnt) {
-    const Cand q = A.cand[o0 + tl_nb * kTile + li];
-    nbx = q.x;
-    nby = q.y;
-    nbz = q.z;
-    nbj = static_cast<int>(q.id);
-  }
-  cx = valid ? px : nan;
-  cy = valid ? py : nan;
-  cz = valid ? pz : nan;
-  cj = valid ? me : -1;
-  sweep(std::true_type{});
-  cx = nbx;
-  cy = nby;
-  cz = nbz;
-  cj = nbj;
-  if (LC_OPEN3) {
-    // the third tile's loads overlap the second tile's sweep
-    nbx = nby = nbz = nan;
-    nbj = -1;
-    if (has_tile && tl_nb2 >= 0 && tl_nb2 * kTile + li < cnt) {
-      const Cand q = A.cand[o0 + tl_nb2 * kTile + li];
+    for (int attempt = 0; attempt < 2; ++attempt) {
+    // opening: the row's own tile (its sources are the candidates, no
+    // shift), then the next tiles of the home cell in sub-cell order
+    const unsigned long long to0 = kStats ? __builtin_readcyclecounter() : 0;
+    if (kStats && attempt == 0) cyc_init = to0 - cyc0;
+    // the next tile's candidates are loaded before the own tile is swept, so
+    // the load overlaps that sweep
+    double nbx = nan, nby = nan, nbz = nan;
+    int nbj = -1;
+    if (has_tile && tl_nb >= 0 && tl_nb * kTile + li < cnt) {
+      const Cand q = A.cand[o0 + tl_nb * kTile + li];
       nbx = q.x;
       nby = q.y;
       nbz = q.z;
       nbj = static_cast<int>(q.id);
     }
-  }
-  if (__ballot(tl_nb >= 0 && has_tile)) sweep(std::false_type{});
-  if (LC_OPEN3) {
+    cx = valid ? px : nan;
+    cy = valid ? py : nan;
+    cz = valid ? pz : nan;
+    cj = valid ? me : -1;
+    sweep(std::true_type{});
     cx = nbx;
     cy = nby;
     cz = nbz;
     cj = nbj;
-    if (__ballot(tl_nb2 >= 0 && has_tile)) sweep(std::false_type{});
-  }
-  if (__ballot(nbuf > 0)) flush();
-  if (kStats) cyc_open += __builtin_readcyclecounter() - to0;
-
-  int lcount = 0;  // entries in this row's list
-  // drain the list: tile i's candidates are loaded while tile i - 1 is
-  // swept
-  auto drain = [&]() LC_INLINE {
-    const unsigned long long t0 = kStats ? __builtin_readcyclecounter() : 0;
-    double nxq = nan, nyq = nan, nzq = nan;
-    int njq = -1, ncode = 13;
-    // half tiles: lanes 0-7 load entry e, lanes 8-15 entry e2
-    auto fetch = [&](int e, int e2) LC_INLINE {
-      nxq = nyq = nzq = nan;
-      njq = -1;
-      ncode = 13;
-      const int el = (kHalf && li >= 8) ? e2 : e;
-      if (el >= 0) {
-        const int2 ent = s_list[wv][rw][el];
-        const int sl0 = ent.x;
-        const int meta = ent.y;
-        ncode = meta & 31;
-        const int m = kHalf ? (li & 7) : li;
-        if (m < (meta >> 5)) {
-          const Cand q = A.cand[sl0 + m];
-          nxq = q.x;
-          nyq = q.y;
-          nzq = q.z;
-          njq = static_cast<int>(q.id);
-        }
+    if (LC_OPEN3) {
+      // the third tile's loads overlap the second tile's sweep
+      nbx = nby = nbz = nan;
+      nbj = -1;
+      if (has_tile && tl_nb2 >= 0 && tl_nb2 * kTile + li < cnt) {
+        const Cand q = A.cand[o0 + tl_nb2 * kTile + li];
+        nbx = q.x;
+        nby = q.y;
+        nbz = q.z;
+        nbj = static_cast<int>(q.id);
       }
-    };
-#if LC_ROW_SELECT && LC_KEYS_REG
-    // nearest first, keys in registers: lane li holds the keys of entries
-    // li, li + 16, ...; taken entries become ~0
-    unsigned kr[NK];
-#pragma unroll
-    for (int j = 0; j < NK; ++j) {
-      const int e = li + 16 * j;
-      kr[j] = e < lcount ? s_key[wv][rw][e] : ~0u;
     }
-    auto take = [&](unsigned key) LC_INLINE {
-      const int e = static_cast<int>(key & 63u);
-#pragma unroll
-      for (int j = 0; j < NK; ++j) kr[j] = (li == (e & 15) && j == (e >> 4)) ? ~0u : kr[j];
-    };
-    auto beyond = [&](unsigned key) LC_INLINE {
-      return static_cast<double>(__uint_as_float(key & ~63u)) > rthr;
-    };
-    // the row's smallest key (and, for half tiles, the second smallest):
-    // -1 once the list is empty or the nearest bound is beyond the row's
-    // threshold (every later entry is then beyond it too)
-    int e2n = -1;
-    auto next = [&](int) LC_INLINE {
-      unsigned m1 = ~0u, m2 = ~0u;
-#pragma unroll
-      for (int j = 0; j < NK; ++j) {
-        const unsigned v = kr[j];
-        if (kHalf) m2 = min(m2, max(m1, v));
-        m1 = min(m1, v);
-      }
-      auto pair_step = [&](auto ctrl) LC_INLINE {
-        constexpr int c = decltype(ctrl)::value;
-        const unsigned b1 = static_cast<unsigned>(
-            __builtin_amdgcn_update_dpp(0, static_cast<int>(m1), c, 0xf, 0xf, true));
-        if (kHalf) {
-          const unsigned b2 = static_cast<unsigned>(
-              __builtin_amdgcn_update_dpp(0, static_cast<int>(m2), c, 0xf, 0xf, true));
-          m2 = min(min(m2, b2), max(m1, b1));
+    if (__ballot(tl_nb >= 0 && has_tile)) sweep(std::false_type{});
+    if (LC_OPEN3) {
+      cx = nbx;
+      cy = nby;
+      cz = nbz;
+      cj = nbj;
+      if (__ballot(tl_nb2 >= 0 && has_tile)) sweep(std::false_type{});
+    }
+    if (__ballot(nbuf > 0)) flush();
+    if (kStats) cyc_open += __builtin_readcyclecounter() - to0;
+
+    int lcount = 0;  // entries in this row's list
+    // drain the list: tile i's candidates are loaded while tile i - 1 is
+    // swept
+    auto drain = [&]() LC_INLINE {
+      const unsigned long long t0 = kStats ? __builtin_readcyclecounter() : 0;
+      double nxq = nan, nyq = nan, nzq = nan;
+      int njq = -1, ncode = 13;
+      // half tiles: lanes 0-7 load entry e, lanes 8-15 entry e2
+      auto fetch = [&](int e, int e2) LC_INLINE {
+        nxq = nyq = nzq = nan;
+        njq = -1;
+        ncode = 13;
+        const int el = (kHalf && li >= 8) ? e2 : e;
+        if (el >= 0) {
+          const int2 ent = s_list[wv][rw][el];
+          const int sl0 = ent.x;
+          const int meta = ent.y;
+          ncode = meta & 31;
+          const int m = kHalf ? (li & 7) : li;
+          if (m < (meta >> 5)) {
+            const Cand q = A.cand[sl0 + m];
+            nxq = q.x;
+            nyq = q.y;
+            nzq = q.z;
+            njq = static_cast<int>(q.id);
+          }
         }
-        m1 = min(m1, b1);
       };
-      pair_step(std::integral_constant<int, 0xB1>{});
-      pair_step(std::integral_constant<int, 0x4E>{});
-      pair_step(std::integral_constant<int, 0x141>{});
-      pair_step(std::integral_constant<int, 0x140>{});
-      e2n = -1;
-      if (m1 == ~0u || beyond(m1)) return -1;
-      take(m1);
-      if (kHalf && m2 != ~0u && !beyond(m2)) {
-        take(m2);
-        e2n = static_cast<int>(m2 & 63u);
-      }
-      return static_cast<int>(m1 & 63u);
-    };
-#elif LC_ROW_SELECT
-    // nearest first: the entry of the row's smallest key, taken out of the
-    // list; -1 once the list is empty or its nearest bound is beyond the
-    // row's threshold (every later entry is then beyond it too)
-    auto next = [&](int) LC_INLINE {
-      unsigned kmin = ~0u;
+#if LC_ROW_SELECT && LC_KEYS_REG
+      // nearest first, keys in registers: lane li holds the keys of entries
+      // li, li + 16, ...; taken entries become ~0
+      unsigned kr[NK];
 #pragma unroll
       for (int j = 0; j < NK; ++j) {
         const int e = li + 16 * j;
-        if (e < lcount) kmin = min(kmin, s_key[wv][rw][e]);
+        kr[j] = e < lcount ? s_key[wv][rw][e] : ~0u;
       }
-      kmin = row_min16_u32(kmin);
-      if (kmin == ~0u) return -1;
-      if (static_cast<double>(__uint_as_float(kmin & ~63u)) > rthr) return -1;
-      const int e = static_cast<int>(kmin & 63u);
-      if (li == (e & 15)) s_key[wv][rw][e] = ~0u;
-      return e;
-    };
+      auto take = [&](unsigned key) LC_INLINE {
+        const int e = static_cast<int>(key & 63u);
+#pragma unroll
+        for (int j = 0; j < NK; ++j) kr[j] = (li == (e & 15) && j == (e >> 4)) ? ~0u : kr[j];
+      };
+      auto beyond = [&](unsigned key) LC_INLINE {
+        return static_cast<double>(__uint_as_float(key & ~63u)) > rthr;
+      };
+      // the row's smallest key (and, for half tiles, the second smallest):
+      // -1 once the list is empty or the nearest bound is beyond the row's
+      // threshold (every later entry is then beyond it too)
+      int e2n = -1;
+      auto next = [&](int) LC_INLINE {
+        unsigned m1 = ~0u, m2 = ~0u;
+#pragma unroll
+        for (int j = 0; j < NK; ++j) {
+          const unsigned v = kr[j];
+          if (kHalf) m2 = min(m2, max(m1, v));
+          m1 = min(m1, v);
+        }
+        auto pair_step = [&](auto ctrl) LC_INLINE {
+          constexpr int c = decltype(ctrl)::value;
+          const unsigned b1 = static_cast<unsigned>(
+              __builtin_amdgcn_update_dpp(0, static_cast<int>(m1), c, 0xf, 0xf, true));
+          if (kHalf) {
+            const unsigned b2 = static_cast<unsigned>(
+                __builtin_amdgcn_update_dpp(0, static_cast<int>(m2), c, 0xf, 0xf, true));
+            m2 = min(min(m2, b2), max(m1, b1));
+          }
+          m1 = min(m1, b1);
+        };
+        pair_step(std::integral_constant<int, 0xB1>{});
+        pair_step(std::integral_constant<int, 0x4E>{});
+        pair_step(std::integral_constant<int, 0x141>{});
+        pair_step(std::integral_constant<int, 0x140>{});
+        e2n = -1;
+        if (m1 == ~0u || beyond(m1)) return -1;
+        take(m1);
+        if (kHalf && m2 != ~0u && !beyond(m2)) {
+          take(m2);
+          e2n = static_cast<int>(m2 & 63u);
+        }
+        return static_cast<int>(m1 & 63u);
+      };
+#elif LC_ROW_SELECT
+      // nearest first: the entry of the row's smallest key, taken out of the
+      // list; -1 once the list is empty or its nearest bound is beyond the
+      // row's threshold (every later entry is then beyond it too)
+      auto next = [&](int) LC_INLINE {
+        unsigned kmin = ~0u;
+#pragma unroll
+        for (int j = 0; j < NK; ++j) {
+          const int e = li + 16 * j;
+          if (e < lcount) kmin = min(kmin, s_key[wv][rw][e]);
+        }
+        kmin = row_min16_u32(kmin);
+        if (kmin == ~0u) return -1;
+        if (static_cast<double>(__uint_as_float(kmin & ~63u)) > rthr) return -1;
+        const int e = static_cast<int>(kmin & 63u);
+        if (li == (e & 15)) s_key[wv][rw][e] = ~0u;
+        return e;
+      };
 #else
-    // in list order
-    auto next = [&](int i) LC_INLINE { return i < lcount ? i : -1; };
+      // in list order
+      auto next = [&](int i) LC_INLINE { return i < lcount ? i : -1; };
 #endif
-    int i = 0;
-    int e = next(i);
+      int i = 0;
+      int e = next(i);
 #if LC_ROW_SELECT && LC_KEYS_REG
-    int e2 = e2n;
+      int e2 = e2n;
 #else
-    int e2 = (kHalf && e >= 0) ? next(i) : -1;
-#endif
-    fetch(e, e2);
-#pragma unroll 1
-    while (__ballot(e >= 0)) {
-      double shx, shy, shz;
-      shift_of(ncode, shx, shy, shz);
-      cx = nxq + shx;
-      cy = nyq + shy;
-      cz = nzq + shz;
-      cj = njq;
-      if (kStats) st_useful += e >= 0 ? 1 : 0;
-      if (kStats && cur_r >= 2) {
-        st_rounds2 += 1;
-        st_useful2 += e >= 0 ? 1 : 0;
-      }
-      const bool swept = e >= 0;
-      i += 1;
-      e = swept ? next(i) : -1;
-#if LC_ROW_SELECT && LC_KEYS_REG
-      e2 = swept ? e2n : -1;
-#else
-      if (kHalf) e2 = e >= 0 ? next(i) : -1;
+      int e2 = (kHalf && e >= 0) ? next(i) : -1;
 #endif
       fetch(e, e2);
-      if (kStats) st_tiles += 1;
-      if (kStats) tile_acc = tile_acc_lo = tile_acc_hi = false;
-      sweep(std::false_type{});
-      if (kStats) {
-        const bool row_any = ((__ballot(tile_acc) >> rbase) & 0xffffull) != 0;
-        const bool row_lo = ((__ballot(tile_acc_lo) >> rbase) & 0xffffull) != 0;
-        const bool row_hi = ((__ballot(tile_acc_hi) >> rbase) & 0xffffull) != 0;
-        st_dead += (swept && !row_any) ? 1 : 0;
-        st_deadh += swept ? (!row_lo ? 1 : 0) + (!row_hi ? 1 : 0) : 0;
-      }
-    }
-    lcount = 0;
-    if (kStats) cyc_drain += __builtin_readcyclecounter() - t0;
-  };
-
-  // the row's shell (rows of one wave may be in different shells under a
-  // depth schedule)
-  int r = 1;
 #pragma unroll 1
-  for (;; ++r) {
-    int ncr = shell_cells(r);
-    cur_r = r;
-    if (kStats && !st_wave2 && __ballot(r >= 2) && attempt == 0) {
-      st_wave2 = 1;
-      cyc_r2 = __builtin_readcyclecounter();
-    }
-    // per-row enumeration state: a chunk of 16 cells, one per lane
-    int cc = 0, p0 = 0, total = 0;
-    int ca0 = 0, cn2 = 0, ct0 = 0, ccode = 13, cnt_c = 0, cpre = 0;
-    bool more = rthr >= 0.0;
-#pragma unroll 1
-    while (true) {
-      // fill the list while every filling row has room for 16 entries
-      const unsigned long long tf0 = kStats ? __builtin_readcyclecounter() : 0;
-#pragma unroll 1
-      while (true) {
-        const bool fill = more && lcount < kFill;
-        if (!__ballot(fill)) break;
-        if (fill) {
-          if (p0 >= total) {
-            // below the predicted depth the next shell's cells follow in
-            // the same pass, with no stop test between
-            if (kDepth && cc >= ncr && r < rp) {
-              r += 1;
-              ncr = shell_cells(r);
-              cc = 0;
-            }
-            if (cc >= ncr) {
-              more = false;
-            } else {
-              // headers of 16 cells, one per lane
-              if (kStats) st_cells += 1;
-              const int ci = cc + li;
-              cn2 = 0;
-              ca0 = ct0 = 0;
-              ccode = 13;
-              if (ci < ncr) {
-                int dx, dy, dz;
-                surface_cell(r, ci, dx, dy, dz);
-                const int jx = hx + dx, jy = hy + dy, jz = hz + dz;
-                const int ux = jx < 0 ? jx + nx : (jx >= nx ? jx - nx : jx);
-                const int uy = jy < 0 ? jy + ny : (jy >= ny ? jy - ny : jy);
-                const int uz = jz < 0 ? jz + nz : (jz >= nz ? jz - nz : jz);
-                const int na = jx < 0 ? -1 : (jx >= nx ? 1 : 0);
-                const int nb = jy < 0 ? -1 : (jy >= ny ? 1 : 0);
-                const int nc = jz < 0 ? -1 : (jz >= nz ? 1 : 0);
-                const int gq = cb + (uz * ny + uy) * nx + ux;
-                ca0 = A.off[gq];
-                cn2 = A.off[gq + 1] - ca0;
-                ct0 = A.tile_off[gq];
-                ccode = (na + 1) * 9 + (nb + 1) * 3 + (nc + 1);
-                if (cn2 > kTile) {
-                  double shx, shy, shz;
-                  shift_of(ccode, shx, shy, shz);
-                  const double sl[3] = {hull[0], hull[2], hull[4]};
-                  const double sh[3] = {hull[1], hull[3], hull[5]};
-                  if (hull_lb(sl, sh, A.cbox[gq], shx, shy, shz) > rthr) cn2 = 0;
-                }
-              }
-              cnt_c = (cn2 + kTile - 1) >> kTileShift;
-              // inclusive prefix of tile counts over the row: DPP row_shr
-              // by 1, 2, 4 and 8 (lanes shifted in from outside the row
-              // read 0), then lane 15's sum broadcast to the row
-              cpre = cnt_c;
-              cpre += __builtin_amdgcn_update_dpp(0, cpre, 0x111, 0xf, 0xf, true);
-              cpre += __builtin_amdgcn_update_dpp(0, cpre, 0x112, 0xf, 0xf, true);
-              cpre += __builtin_amdgcn_update_dpp(0, cpre, 0x114, 0xf, 0xf, true);
-              cpre += __builtin_amdgcn_update_dpp(0, cpre, 0x118, 0xf, 0xf, true);
-              total = row_bcast<15>(cpre);
-              p0 = 0;
-              cc += 16;
-            }
-          }
-          if (more && p0 < total) {
-            // 16 tiles of the chunk, one per lane
-            const int p = p0 + li;
-            const bool pv = p < total;
-            int own = 0;
-#pragma unroll
-            for (int step2 = 8; step2 >= 1; step2 >>= 1) {
-              const int probe = own + step2 - 1;
-              const int pr = __shfl(cpre, rbase + (probe < 16 ? probe : 15));
-              if (probe < 16 && pr <= p) own += step2;
-            }
-            own = own < 16 ? own : 15;
-            const int oa0 = __shfl(ca0, rbase + own);
-            const int on2 = __shfl(cn2, rbase + own);
-            const int ot0 = __shfl(ct0, rbase + own);
-            const int ocode = __shfl(ccode, rbase + own);
-            const int oex = __shfl(cpre - cnt_c, rbase + own);
-            // the home cell's tiles go outward from the row's own tile in
-            // sub-cell order (tl, tl + 1, tl - 1, ...): the nearest first,
-            // so the thresholds tighten before the far tiles are tested
-            const bool home = r == 1 && cc == 16 && own == 0;
-            const int t = (LC_HOME_SPIRAL && home) ? spiral_tile(p - oex, tl, nt_home)
-                                                   : p - oex;
-            const bool opening = home && (t == tl || t == tl_nb || t == tl_nb2);
-            bool need = false;
-            double lbt = inf;  // lower bound of the tile's distance to the row's sources
-            double bx0 = 0, bx1 = 0, by0 = 0, by1 = 0, bz0 = 0, bz1 = 0;
-            if (pv && !opening) {
-              double shx, shy, shz;
-              shift_of(ocode, shx, shy, shz);
-              const Box b = A.tbox[ot0 + t];
-              bx0 = b.lx + shx;
-              bx1 = b.hx + shx;
-              by0 = b.ly + shy;
-              by1 = b.hy + shy;
-              bz0 = b.lz + shz;
-              bz1 = b.hz + shz;
-              if (kStats) st_tests += 1;
-              const double gx = fmax(fmax(bx0 - hull[1], hull[0] - bx1), 0.0);
-              const double gy = fmax(fmax(by0 - hull[3], hull[2] - by1), 0.0);
-              const double gz = fmax(fmax(bz0 - hull[5], hull[4] - bz1), 0.0);
-              lbt = gx * gx + gy * gy + gz * gz;
-              need = lbt <= rthr;
-            }
-            if constexpr (kHalf) {
-              // exact per-source test of each half of the tiles that passed
-              // the row box, with the half tiles' own boxes
-              bool need_a = false, need_b = false;
-              double lb_a = inf, lb_b = inf;
-              const int e = on2 - t * kTile;  // members of the tile
-              if (__ballot(need)) {
-                if (kStats) st_exact += 1;
-                double ax0 = 0, ax1 = 0, ay0 = 0, ay1 = 0, az0 = 0, az1 = 0;
-                double cx0 = 0, cx1 = 0, cy0 = 0, cy1 = 0, cz0 = 0, cz1 = 0;
-                if (need) {
-                  double shx, shy, shz;
-                  shift_of(ocode, shx, shy, shz);
-                  const Box ha = A.hbox[2 * (ot0 + t)];
-                  const Box hb = A.hbox[2 * (ot0 + t) + 1];
-                  ax0 = ha.lx + shx;
-                  ax1 = ha.hx + shx;
-                  ay0 = ha.ly + shy;
-                  ay1 = ha.hy + shy;
-                  az0 = ha.lz + shz;
-                  az1 = ha.hz + shz;
-                  cx0 = hb.lx + shx;
-                  cx1 = hb.hx + shx;
-                  cy0 = hb.ly + shy;
-                  cy1 = hb.hy + shy;
-                  cz0 = hb.lz + shz;
-                  cz1 = hb.hz + shz;
-                }
-                const double tq = live ? thr : -inf;
-                bool any_a = false, any_b = false;
-                auto probe_src = [&](auto I) LC_INLINE {
-                  constexpr int ii = decltype(I)::value;
-                  const double qx = row_bcast<ii>(px);
-                  const double qy = row_bcast<ii>(py);
-                  const double qz = row_bcast<ii>(pz);
-                  const double qt = row_bcast<ii>(tq);
-                  {
-                    const double gx = fmax(fmax(ax0 - qx, qx - ax1), 0.0);
-                    const double gy = fmax(fmax(ay0 - qy, qy - ay1), 0.0);
-                    const double gz = fmax(fmax(az0 - qz, qz - az1), 0.0);
-                    const double ls = gx * gx + gy * gy + gz * gz;
-                    any_a |= ls <= qt;
-                    lb_a = key_min(lb_a, ls <= qt ? ls : inf);
-                  }
-                  {
-                    const double gx = fmax(fmax(cx0 - qx, qx - cx1), 0.0);
-                    const double gy = fmax(fmax(cy0 - qy, qy - cy1), 0.0);
-                    const double gz = fmax(fmax(cz0 - qz, qz - cz1), 0.0);
-                    const double ls = gx * gx + gy * gy + gz * gz;
-                    any_b |= ls <= qt;
-                    lb_b = key_min(lb_b, ls <= qt ? ls : inf);
-                  }
-                };
-                probe_src(std::integral_constant<int, 0>{});
-                probe_src(std::integral_constant<int, 1>{});
-                probe_src(std::integral_constant<int, 2>{});
-                probe_src(std::integral_constant<int, 3>{});
-                probe_src(std::integral_constant<int, 4>{});
-                probe_src(std::integral_constant<int, 5>{});
-                probe_src(std::integral_constant<int, 6>{});
-                probe_src(std::integral_constant<int, 7>{});
-                probe_src(std::integral_constant<int, 8>{});
-                probe_src(std::integral_constant<int, 9>{});
-                probe_src(std::integral_constant<int, 10>{});
-                probe_src(std::integral_constant<int, 11>{});
-                probe_src(std::integral_constant<int, 12>{});
-                probe_src(std::integral_constant<int, 13>{});
-                probe_src(std::integral_constant<int, 14>{});
-                probe_src(std::integral_constant<int, 15>{});
-                // a tile of 8 or fewer has an empty second half, whose
-                // (+inf, -inf) box passes an infinite threshold
-                need_a = need && any_a;
-                need_b = need && any_b && e > 8;
-              }
-              const unsigned rma = static_cast<unsigned>(__ballot(need_a) >> rbase) & 0xffffu;
-              const unsigned rmb = static_cast<unsigned>(__ballot(need_b) >> rbase) & 0xffffu;
-              const unsigned below = (1u << li) - 1u;
-              if (need_a) {
-                const int pos = lcount + __popc(rma & below);
-                s_list[wv][rw][pos] = make_int2(oa0 + t * kTile, ocode | ((e < 8 ? e : 8) << 5));
-                s_key[wv][rw][pos] = (__float_as_uint(__double2float_rd(lb_a)) & ~63u) |
-                                     static_cast<unsigned>(pos);
-              }
-              if (need_b) {
-                const int pos = lcount + __popc(rma) + __popc(rmb & below);
-                s_list[wv][rw][pos] = make_int2(oa0 + t * kTile + 8, ocode | ((e - 8) << 5));
-                s_key[wv][rw][pos] = (__float_as_uint(__double2float_rd(lb_b)) & ~63u) |
-                                     static_cast<unsigned>(pos);
-              }
-              lcount += __popc(rma) + __popc(rmb);
-            } else {
-            // exact per-source test of the tiles that passed the row box
-              if (__ballot(need)) {
-                if (kStats) st_exact += 1;
-                const double tq = live ? thr : -inf;
-                bool any = false;
-                double lbs = inf;  // smallest per-source bound (a tighter key)
-                auto probe_src = [&](auto I) LC_INLINE {
-                  constexpr int ii = decltype(I)::value;
-                  const double qx = row_bcast<ii>(px);
-                  const double qy = row_bcast<ii>(py);
-                  const double qz = row_bcast<ii>(pz);
-                  const double qt = row_bcast<ii>(tq);
-                  const double gx = fmax(fmax(bx0 - qx, qx - bx1), 0.0);
-                  const double gy = fmax(fmax(by0 - qy, qy - by1), 0.0);
-                  const double gz = fmax(fmax(bz0 - qz, qz - bz1), 0.0);
-                  const double ls = gx * gx + gy * gy + gz * gz;
-                  any |= ls <= qt;
-                  // only sources that can still take the tile bound its key
-                  if (LC_ROW_SELECT) lbs = key_min(lbs, ls <= qt ? ls : inf);
-                };
-                probe_src(std::integral_constant<int, 0>{});
-                probe_src(std::integral_constant<int, 1>{});
-                probe_src(std::integral_constant<int, 2>{});
-                probe_src(std::integral_constant<int, 3>{});
-                probe_src(std::integral_constant<int, 4>{});
-                probe_src(std::integral_constant<int, 5>{});
-                probe_src(std::integral_constant<int, 6>{});
-                probe_src(std::integral_constant<int, 7>{});
-                probe_src(std::integral_constant<int, 8>{});
-                probe_src(std::integral_constant<int, 9>{});
-                probe_src(std::integral_constant<int, 10>{});
-                probe_src(std::integral_constant<int, 11>{});
-                probe_src(std::integral_constant<int, 12>{});
-                probe_src(std::integral_constant<int, 13>{});
-                probe_src(std::integral_constant<int, 14>{});
-                probe_src(std::integral_constant<int, 15>{});
-                need = need && any;
-                if (LC_ROW_SELECT) lbt = key_max(lbt, lbs);
-              }
-              const unsigned long long bal = __ballot(need);
-              const unsigned rm = static_cast<unsigned>(bal >> rbase) & 0xffffu;
-              if (need) {
-                const int pos = lcount + __popc(rm & ((1u << li) - 1u));
-                const int e = on2 - t * kTile;
-                s_list[wv][rw][pos] = make_int2(oa0 + t * kTile, ocode | ((e < kTile ? e : kTile) << 5));
-                if (LC_ROW_SELECT) {
-                  s_key[wv][rw][pos] =
-                      (__float_as_uint(__double2float_rd(lbt)) & ~63u) | static_cast<unsigned>(pos);
-                }
-              }
-  lcount += __popc(rm);
-            }
-            p0 += 16;
-          }
+      while (__ballot(e >= 0)) {
+        double shx, shy, shz;
+        shift_of(ncode, shx, shy, shz);
+        cx = nxq + shx;
+        cy = nyq + shy;
+        cz = nzq + shz;
+        cj = njq;
+        if (kStats) st_useful += e >= 0 ? 1 : 0;
+        if (kStats && cur_r >= 2) {
+          st_rounds2 += 1;
+          st_useful2 += e >= 0 ? 1 : 0;
+        }
+        const bool swept = e >= 0;
+        i += 1;
+        e = swept ? next(i) : -1;
+#if LC_ROW_SELECT && LC_KEYS_REG
+        e2 = swept ? e2n : -1;
+#else
+        if (kHalf) e2 = e >= 0 ? next(i) : -1;
+#endif
+        fetch(e, e2);
+        if (kStats) st_tiles += 1;
+        if (kStats) tile_acc = tile_acc_lo = tile_acc_hi = false;
+        sweep(std::false_type{});
+        if (kStats) {
+          const bool row_any = ((__ballot(tile_acc) >> rbase) & 0xffffull) != 0;
+          const bool row_lo = ((__ballot(tile_acc_lo) >> rbase) & 0xffffull) != 0;
+          const bool row_hi = ((__ballot(tile_acc_hi) >> rbase) & 0xffffull) != 0;
+          st_dead += (swept && !row_any) ? 1 : 0;
+          st_deadh += swept ? (!row_lo ? 1 : 0) + (!row_hi ? 1 : 0) : 0;
         }
       }
-      if (kStats) cyc_fill += __builtin_readcyclecounter() - tf0;
-      if (__ballot(lcount > 0)) drain();
-      if (!__ballot(more)) break;
-    }
-    if (__ballot(nbuf > 0)) flush();
-    // stop test, src/knearest.rs:466-471
-    const double bound = static_cast<double>(r) * d.cell_min;
-    if (live && h.j[KC - 1] != kEmptyJ) {
-      // a full heap holds only d2 <= thr_cap, and everything skipped or
-      // rejected lies beyond it: the heap is the exact top-k of the shells
-      // walked so far
-      if (as_f64(h.d[KC - 1]) <= bound * bound) {
-        live = false;
-        done_reach = r;
+      lcount = 0;
+      if (kStats) cyc_drain += __builtin_readcyclecounter() - t0;
+    };
+
+    // the row's shell (rows of one wave may be in different shells under a
+    // depth schedule)
+    int r = 1;
+#pragma unroll 1
+    for (;; ++r) {
+      int ncr = shell_cells(r);
+      cur_r = r;
+      if (kStats && !st_wave2 && __ballot(r >= 2) && attempt == 0) {
+        st_wave2 = 1;
+        cyc_r2 = __builtin_readcyclecounter();
       }
-    } else if (live && thr_cap < bound * bound) {
-      // not full: the plain walk might have filled its heap with candidates
-      // beyond the cap and stopped here; only an uncapped walk can tell
-      live = false;
-      cap_fail = true;
+      // per-row enumeration state: a chunk of 16 cells, one per lane
+      int cc = 0, p0 = 0, total = 0;
+      int ca0 = 0, cn2 = 0, ct0 = 0, ccode = 13, cnt_c = 0, cpre = 0;
+      bool more = rthr >= 0.0;
+#pragma unroll 1
+      while (true) {
+        // fill the list while every filling row has room for 16 entries
+        const unsigned long long tf0 = kStats ? __builtin_readcyclecounter() : 0;
+#pragma unroll 1
+        while (true) {
+          const bool fill = more && lcount < kFill;
+          if (!__ballot(fill)) break;
+          if (fill) {
+            if (p0 >= total) {
+              // below the predicted depth the next shell's cells follow in
+              // the same pass, with no stop test between
+              if (kDepth && cc >= ncr && r < rp) {
+                r += 1;
+                ncr = shell_cells(r);
+                cc = 0;
+              }
+              if (cc >= ncr) {
+                more = false;
+              } else {
+                // headers of 16 cells, one per lane
+                if (kStats) st_cells += 1;
+                const int ci = cc + li;
+                cn2 = 0;
+                ca0 = ct0 = 0;
+                ccode = 13;
+                if (ci < ncr) {
+                  int dx, dy, dz;
+                  surface_cell(r, ci, dx, dy, dz);
+                  const int jx = hx + dx, jy = hy + dy, jz = hz + dz;
+                  const int ux = jx < 0 ? jx + nx : (jx >= nx ? jx - nx : jx);
+                  const int uy = jy < 0 ? jy + ny : (jy >= ny ? jy - ny : jy);
+                  const int uz = jz < 0 ? jz + nz : (jz >= nz ? jz - nz : jz);
+                  const int na = jx < 0 ? -1 : (jx >= nx ? 1 : 0);
+                  const int nb = jy < 0 ? -1 : (jy >= ny ? 1 : 0);
+                  const int nc = jz < 0 ? -1 : (jz >= nz ? 1 : 0);
+                  const int gq = cb + (uz * ny + uy) * nx + ux;
+                  ca0 = A.off[gq];
+                  cn2 = A.off[gq + 1] - ca0;
+                  ct0 = A.tile_off[gq];
+                  ccode = (na + 1) * 9 + (nb + 1) * 3 + (nc + 1);
+                  if (cn2 > kTile) {
+                    double shx, shy, shz;
+                    shift_of(ccode, shx, shy, shz);
+                    const double sl[3] = {hull[0], hull[2], hull[4]};
+                    const double sh[3] = {hull[1], hull[3], hull[5]};
+                    if (hull_lb(sl, sh, A.cbox[gq], shx, shy, shz) > rthr) cn2 = 0;
+                  }
+                }
+                cnt_c = (cn2 + kTile - 1) >> kTileShift;
+                // inclusive prefix of tile counts over the row: DPP row_shr
+                // by 1, 2, 4 and 8 (lanes shifted in from outside the row
+                // read 0), then lane 15's sum broadcast to the row
+                cpre = cnt_c;
+                cpre += __builtin_amdgcn_update_dpp(0, cpre, 0x111, 0xf, 0xf, true);
+                cpre += __builtin_amdgcn_update_dpp(0, cpre, 0x112, 0xf, 0xf, true);
+                cpre += __builtin_amdgcn_update_dpp(0, cpre, 0x114, 0xf, 0xf, true);
+                cpre += __builtin_amdgcn_update_dpp(0, cpre, 0x118, 0xf, 0xf, true);
+                total = row_bcast<15>(cpre);
+                p0 = 0;
+                cc += 16;
+              }
+            }
+            if (more && p0 < total) {
+              // 16 tiles of the chunk, one per lane
+              const int p = p0 + li;
+              const bool pv = p < total;
+              int own = 0;
+#pragma unroll
+              for (int step2 = 8; step2 >= 1; step2 >>= 1) {
+                const int probe = own + step2 - 1;
+                const int pr = __shfl(cpre, rbase + (probe < 16 ? probe : 15));
+                if (probe < 16 && pr <= p) own += step2;
+              }
+              own = own < 16 ? own : 15;
+              const int oa0 = __shfl(ca0, rbase + own);
+              const int on2 = __shfl(cn2, rbase + own);
+              const int ot0 = __shfl(ct0, rbase + own);
+              const int ocode = __shfl(ccode, rbase + own);
+              const int oex = __shfl(cpre - cnt_c, rbase + own);
+              // the home cell's tiles go outward from the row's own tile in
+              // sub-cell order (tl, tl + 1, tl - 1, ...): the nearest first,
+              // so the thresholds tighten before the far tiles are tested
+              const bool home = r == 1 && cc == 16 && own == 0;
+              const int t = (LC_HOME_SPIRAL && home) ? spiral_tile(p - oex, tl, nt_home)
+                                                     : p - oex;
+              const bool opening = home && (t == tl || t == tl_nb || t == tl_nb2);
+              bool need = false;
+              double lbt = inf;  // lower bound of the tile's distance to the row's sources
+              double bx0 = 0, bx1 = 0, by0 = 0, by1 = 0, bz0 = 0, bz1 = 0;
+              if (pv && !opening) {
+                double shx, shy, shz;
+                shift_of(ocode, shx, shy, shz);
+                const Box b = A.tbox[ot0 + t];
+                bx0 = b.lx + shx;
+                bx1 = b.hx + shx;
+                by0 = b.ly + shy;
+                by1 = b.hy + shy;
+                bz0 = b.lz + shz;
+                bz1 = b.hz + shz;
+                if (kStats) st_tests += 1;
+                const double gx = fmax(fmax(bx0 - hull[1], hull[0] - bx1), 0.0);
+                const double gy = fmax(fmax(by0 - hull[3], hull[2] - by1), 0.0);
+                const double gz = fmax(fmax(bz0 - hull[5], hull[4] - bz1), 0.0);
+                lbt = gx * gx + gy * gy + gz * gz;
+                need = lbt <= rthr;
+              }
+              if constexpr (kHalf) {
+                // exact per-source test of each half of the tiles that passed
+                // the row box, with the half tiles' own boxes
+                bool need_a = false, need_b = false;
+                double lb_a = inf, lb_b = inf;
+                const int e = on2 - t * kTile;  // members of the tile
+                if (__ballot(need)) {
+                  if (kStats) st_exact += 1;
+                  double ax0 = 0, ax1 = 0, ay0 = 0, ay1 = 0, az0 = 0, az1 = 0;
+                  double cx0 = 0, cx1 = 0, cy0 = 0, cy1 = 0, cz0 = 0, cz1 = 0;
+                  if (need) {
+                    double shx, shy, shz;
+                    shift_of(ocode, shx, shy, shz);
+                    const Box ha = A.hbox[2 * (ot0 + t)];
+                    const Box hb = A.hbox[2 * (ot0 + t) + 1];
+                    ax0 = ha.lx + shx;
+                    ax1 = ha.hx + shx;
+                    ay0 = ha.ly + shy;
+                    ay1 = ha.hy + shy;
+                    az0 = ha.lz + shz;
+                    az1 = ha.hz + shz;
+                    cx0 = hb.lx + shx;
+                    cx1 = hb.hx + shx;
+                    cy0 = hb.ly + shy;
+                    cy1 = hb.hy + shy;
+                    cz0 = hb.lz + shz;
+                    cz1 = hb.hz + shz;
+                  }
+                  const double tq = live ? thr : -inf;
+                  bool any_a = false, any_b = false;
+                  auto probe_src = [&](auto I) LC_INLINE {
+                    constexpr int ii = decltype(I)::value;
+                    const double qx = row_bcast<ii>(px);
+                    const double qy = row_bcast<ii>(py);
+                    const double qz = row_bcast<ii>(pz);
+                    const double qt = row_bcast<ii>(tq);
+                    {
+                      const double gx = fmax(fmax(ax0 - qx, qx - ax1), 0.0);
+                      const double gy = fmax(fmax(ay0 - qy, qy - ay1), 0.0);
+                      const double gz = fmax(fmax(az0 - qz, qz - az1), 0.0);
+                      const double ls = gx * gx + gy * gy + gz * gz;
+                      any_a |= ls <= qt;
+                      lb_a = key_min(lb_a, ls <= qt ? ls : inf);
+                    }
+                    {
+                      const double gx = fmax(fmax(cx0 - qx, qx - cx1), 0.0);
+                      const double gy = fmax(fmax(cy0 - qy, qy - cy1), 0.0);
+                      const double gz = fmax(fmax(cz0 - qz, qz - cz1), 0.0);
+                      const double ls = gx * gx + gy * gy + gz * gz;
+                      any_b |= ls <= qt;
+                      lb_b = key_min(lb_b, ls <= qt ? ls : inf);
+                    }
+                  };
+                  probe_src(std::integral_constant<int, 0>{});
+                  probe_src(std::integral_constant<int, 1>{});
+                  probe_src(std::integral_constant<int, 2>{});
+                  probe_src(std::integral_constant<int, 3>{});
+                  probe_src(std::integral_constant<int, 4>{});
+                  probe_src(std::integral_constant<int, 5>{});
+                  probe_src(std::integral_constant<int, 6>{});
+                  probe_src(std::integral_constant<int, 7>{});
+                  probe_src(std::integral_constant<int, 8>{});
+                  probe_src(std::integral_constant<int, 9>{});
+                  probe_src(std::integral_constant<int, 10>{});
+                  probe_src(std::integral_constant<int, 11>{});
+                  probe_src(std::integral_constant<int, 12>{});
+                  probe_src(std::integral_constant<int, 13>{});
+                  probe_src(std::integral_constant<int, 14>{});
+                  probe_src(std::integral_constant<int, 15>{});
+                  // a tile of 8 or fewer has an empty second half, whose
+                  // (+inf, -inf) box passes an infinite threshold
+                  need_a = need && any_a;
+                  need_b = need && any_b && e > 8;
+                }
+                const unsigned rma = static_cast<unsigned>(__ballot(need_a) >> rbase) & 0xffffu;
+                const unsigned rmb = static_cast<unsigned>(__ballot(need_b) >> rbase) & 0xffffu;
+                const unsigned below = (1u << li) - 1u;
+                if (need_a) {
+                  const int pos = lcount + __popc(rma & below);
+                  s_list[wv][rw][pos] = make_int2(oa0 + t * kTile, ocode | ((e < 8 ? e : 8) << 5));
+                  s_key[wv][rw][pos] = (__float_as_uint(__double2float_rd(lb_a)) & ~63u) |
+                                       static_cast<unsigned>(pos);
+                }
+                if (need_b) {
+                  const int pos = lcount + __popc(rma) + __popc(rmb & below);
+                  s_list[wv][rw][pos] = make_int2(oa0 + t * kTile + 8, ocode | ((e - 8) << 5));
+                  s_key[wv][rw][pos] = (__float_as_uint(__double2float_rd(lb_b)) & ~63u) |
+                                       static_cast<unsigned>(pos);
+                }
+                lcount += __popc(rma) + __popc(rmb);
+              } else {
+              // exact per-source test of the tiles that passed the row box
+                if (__ballot(need)) {
+                  if (kStats) st_exact += 1;
+                  const double tq = live ? thr : -inf;
+                  bool any = false;
+                  double lbs = inf;  // smallest per-source bound (a tighter key)
+                  auto probe_src = [&](auto I) LC_INLINE {
+                    constexpr int ii = decltype(I)::value;
+                    const double qx = row_bcast<ii>(px);
+                    const double qy = row_bcast<ii>(py);
+                    const double qz = row_bcast<ii>(pz);
+                    const double qt = row_bcast<ii>(tq);
+                    const double gx = fmax(fmax(bx0 - qx, qx - bx1), 0.0);
+                    const double gy = fmax(fmax(by0 - qy, qy - by1), 0.0);
+                    const double gz = fmax(fmax(bz0 - qz, qz - bz1), 0.0);
+                    const double ls = gx * gx + gy * gy + gz * gz;
+                    any |= ls <= qt;
+                    // only sources that can still take the tile bound its key
+                    if (LC_ROW_SELECT) lbs = key_min(lbs, ls <= qt ? ls : inf);
+                  };
+                  probe_src(std::integral_constant<int, 0>{});
+                  probe_src(std::integral_constant<int, 1>{});
+                  probe_src(std::integral_constant<int, 2>{});
+                  probe_src(std::integral_constant<int, 3>{});
+                  probe_src(std::integral_constant<int, 4>{});
+                  probe_src(std::integral_constant<int, 5>{});
+                  probe_src(std::integral_constant<int, 6>{});
+                  probe_src(std::integral_constant<int, 7>{});
+                  probe_src(std::integral_constant<int, 8>{});
+                  probe_src(std::integral_constant<int, 9>{});
+                  probe_src(std::integral_constant<int, 10>{});
+                  probe_src(std::integral_constant<int, 11>{});
+                  probe_src(std::integral_constant<int, 12>{});
+                  probe_src(std::integral_constant<int, 13>{});
+                  probe_src(std::integral_constant<int, 14>{});
+                  probe_src(std::integral_constant<int, 15>{});
+                  need = need && any;
+                  if (LC_ROW_SELECT) lbt = key_max(lbt, lbs);
+                }
+                const unsigned long long bal = __ballot(need);
+                const unsigned rm = static_cast<unsigned>(bal >> rbase) & 0xffffu;
+                if (need) {
+                  const int pos = lcount + __popc(rm & ((1u << li) - 1u));
+                  const int e = on2 - t * kTile;
+                  s_list[wv][rw][pos] = make_int2(oa0 + t * kTile, ocode | ((e < kTile ? e : kTile) << 5));
+                  if (LC_ROW_SELECT) {
+                    s_key[wv][rw][pos] =
+                        (__float_as_uint(__double2float_rd(lbt)) & ~63u) | static_cast<unsigned>(pos);
+                  }
+                }
+    lcount += __popc(rm);
+              }
+              p0 += 16;
+            }
+          }
+        }
+        if (kStats) cyc_fill += __builtin_readcyclecounter() - tf0;
+        if (__ballot(lcount > 0)) drain();
+        if (!__ballot(more)) break;
+      }
+      if (__ballot(nbuf > 0)) flush();
+      // stop test, src/knearest.rs:466-471
+      const double bound = static_cast<double>(r) * d.cell_min;
+      if (live && h.j[KC - 1] != kEmptyJ) {
+        // a full heap holds only d2 <= thr_cap, and everything skipped or
+        // rejected lies beyond it: the heap is the exact top-k of the shells
+        // walked so far
+        if (as_f64(h.d[KC - 1]) <= bound * bound) {
+          live = false;
+          done_reach = r;
+        }
+      } else if (live && thr_cap < bound * bound) {
+        // not full: the plain walk might have filled its heap with candidates
+        // beyond the cap and stopped here; only an uncapped walk can tell
+        live = false;
+        cap_fail = true;
+      }
+      // beyond r_safe the stencil wraps onto itself: exact walk
+      if (live && r + 1 > d.r_safe) live = false;
+      if (!live) thr = -inf;
+      rthr = row_max16(live ? thr : -inf);
+      if (!__ballot(live)) break;
     }
-    // beyond r_safe the stencil wraps onto itself: exact walk
-    if (live && r + 1 > d.r_safe) live = false;
-    if (!live) thr = -inf;
+    const bool row_fail = ((__ballot(cap_fail) >> rbase) & 0xffffull) != 0;
+    if (!__ballot(row_fail)) break;
+    if (row_fail && li == 0 && A.retry_count != nullptr) atomicAdd(A.retry_count, 1);
+    // the failed rows start over without the cap; every other row is done
+    if (row_fail) {
+      top_init<KC>(h, k);
+      min_rej = inf;
+      thr_cap = inf;
+      rp = 1;
+      live = valid;
+      thr = live ? inf : -inf;
+      done_reach = 0;
+      cap_fail = false;
+    }
     rthr = row_max16(live ? thr : -inf);
-    if (!__ballot(live)) break;
-  }
-  const bool row_fail = ((__ballot(cap_fail) >> rbase) & 0xffffull) != 0;
-  if (!__ballot(row_fail)) break;
-  if (row_fail && li == 0 && A.retry_count != nullptr) atomicAdd(A.retry_count, 1);
-  // the failed rows start over without the cap; every other row is done
-  if (row_fail) {
-    top_init<KC>(h, k);
-    min_rej = inf;
-    thr_cap = inf;
-    rp = 1;
-    live = valid;
-    thr = live ? inf : -inf;
-    done_reach = 0;
-    cap_fail = false;
-  }
-  rthr = row_max16(live ? thr : -inf);
-  }
-  if (kStats) {
-    stat_add(A.stats, kStatRowTileTests, st_tests);
-    stat_add(A.stats, kStatRowMerges, st_merges);
-    if (valid) stat_add(A.stats, kStatRowSources, 1);
-    if (li == 0) {
-      stat_add(A.stats, kStatRowTiles, st_tiles);
-      stat_add(A.stats, kStatRowRefines, st_exact);
-      stat_add(A.stats, kStatRowUseful, st_useful);
     }
-    if (lane == 0) {
-      stat_add(A.stats, kStatRowRounds, st_rounds);
-      stat_add(A.stats, kStatRowCells, st_cells);
-      stat_add(A.stats, kStatRowCycFill, cyc_fill);
-      stat_add(A.stats, kStatRowCycDrain, cyc_drain);
-      stat_add(A.stats, kStatRowCycMerge, cyc_merge);
-      stat_add(A.stats, kStatRowCycAll, __builtin_readcyclecounter() - cyc0);
-      stat_add(A.stats, kStatRowFlushes, st_flush);
-      stat_add(A.stats, kStatRowWavesR2, st_wave2);
-      stat_add(A.stats, kStatRowRoundsR2, st_rounds2);
-      if (st_wave2) stat_add(A.stats, kStatRowCycR2, __builtin_readcyclecounter() - cyc_r2);
-      stat_add(A.stats, kStatRowCycOpen, cyc_open);
-      stat_add(A.stats, kStatRowCycInit, cyc_init);
+    if (kStats) {
+      stat_add(A.stats, kStatRowTileTests, st_tests);
+      stat_add(A.stats, kStatRowMerges, st_merges);
+      if (valid) stat_add(A.stats, kStatRowSources, 1);
+      if (li == 0) {
+        stat_add(A.stats, kStatRowTiles, st_tiles);
+        stat_add(A.stats, kStatRowRefines, st_exact);
+        stat_add(A.stats, kStatRowUseful, st_useful);
+      }
+      if (lane == 0) {
+        stat_add(A.stats, kStatRowRounds, st_rounds);
+        stat_add(A.stats, kStatRowCells, st_cells);
+        stat_add(A.stats, kStatRowCycFill, cyc_fill);
+        stat_add(A.stats, kStatRowCycDrain, cyc_drain);
+        stat_add(A.stats, kStatRowCycMerge, cyc_merge);
+        stat_add(A.stats, kStatRowCycAll, __builtin_readcyclecounter() - cyc0);
+        stat_add(A.stats, kStatRowFlushes, st_flush);
+        stat_add(A.stats, kStatRowWavesR2, st_wave2);
+        stat_add(A.stats, kStatRowRoundsR2, st_rounds2);
+        if (st_wave2) stat_add(A.stats, kStatRowCycR2, __builtin_readcyclecounter() - cyc_r2);
+        stat_add(A.stats, kStatRowCycOpen, cyc_open);
+        stat_add(A.stats, kStatRowCycInit, cyc_init);
+      }
+      if (li == 0) stat_add(A.stats, kStatRowUsefulR2, st_useful2);
+      if (li == 0) stat_add(A.stats, kStatRowDead, st_dead);
+      if (li == 0) stat_add(A.stats, kStatRowDeadHalf, st_deadh);
+      stat_add(A.stats, kStatRowAccepts, st_acc);
     }
-    if (li == 0) stat_add(A.stats, kStatRowUsefulR2, st_useful2);
-    if (li == 0) stat_add(A.stats, kStatRowDead, st_dead);
-    if (li == 0) stat_add(A.stats, kStatRowDeadHalf, st_deadh);
-    stat_add(A.stats, kStatRowAccepts, st_acc);
-  }
-  if (!valid) return;
-  const size_t row = static_cast<size_t>(me) * k;
-  if (done_reach == 0 ||
-      !top_emit<KC>(h, k, min_rej, d.out_nn, d.out_d2, row)) {
-    defer_source(s, A.defer_list, A.defer_count);
-    return;
-  }
-  if (d.out_reach) d.out_reach[me] = done_reach;
+    if (!valid) continue;
+    const size_t row = static_cast<size_t>(me) * k;
+    if (done_reach == 0 ||
+        !top_emit<KC>(h, k, min_rej, d.out_nn, d.out_d2, row)) {
+      defer_source(s, A.defer_list, A.defer_count);
+      continue;
+    }
+    if (d.out_reach) d.out_reach[me] = done_reach;
+  } while (kPersist);
 }
 
 // Queue every active source of the frames that have no register-heap
@@ -3428,7 +3448,7 @@ void buffer_bytes(const Sizes& z, size_t (&b)[kNumBufs]) {
       sizeof(long long) * (F + 1),                   // fcb
       sizeof(int) * C,                               // count
       sizeof(unsigned long long) * 2 * static_cast<size_t>(z.tiles),  // status
-      sizeof(int) * 4,                               // counters
+      sizeof(int) * 12,                              // counters (8..11: walk queues)
       sizeof(int) * (C + 1),                         // off
       sizeof(int) * (C + 1),                         // tile_off
       sizeof(int) * (C + 1),                         // item_off
@@ -3510,6 +3530,23 @@ void launch_prep(const Sizes& z, const Bufs& B, hipStream_t st, hipEvent_t* ev) 
 }
 
 // Walk arguments over the call's buffers, plain schedule, no counters.
+// Compute units of the current device (looked up once per device).
+int device_cus() {
+  static std::mutex mu;
+  static int cus[64] = {};
+  int dev = 0;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= 64) return 256;
+  std::lock_guard<std::mutex> lock(mu);
+  if (cus[dev] == 0) {
+    int v = 0;
+    check(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev),
+          "hipDeviceGetAttribute");
+    cus[dev] = v > 0 ? v : 256;
+  }
+  return cus[dev];
+}
+
 WalkArgs walk_args(const Sizes& z, const Bufs& B, int dense_min) {
   WalkArgs wa;
   std::memset(&wa, 0, sizeof wa);
@@ -3542,10 +3579,21 @@ WalkArgs walk_args(const Sizes& z, const Bufs& B, int dense_min) {
 void launch_walk(const Sizes& z, const Bufs& B, const WalkArgs& wa,
                  const bool (&any_fast)[33], bool any_exact_frame, hipStream_t st,
                  hipEvent_t* ev) {
-  const bool row_mode = wa.dense_min == kRowWalk;
+  const bool row_mode = wa.dense_min == kRowWalk || wa.dense_min == kRowWalkPersist;
   const unsigned sparse_blocks = blocks_for(z.ntot, 256);
   // row walk: one wave per 4 source tiles, frames on wave boundaries
   const unsigned row_blocks = blocks_for(z.tcap / 4 + z.F + 1, 4);
+  // persistent row walk: as many blocks as the GPU holds at once, each
+  // bucket's waves taking groups from their own counter (cleared here by a
+  // fill kernel, so the sequence can be captured into a graph)
+  const bool persist = row_mode && wa.queue != nullptr && wa.stats == nullptr;
+  WalkArgs pa = wa;
+  pa.n_waves = static_cast<int>(row_blocks) * 4;
+  const unsigned cus = static_cast<unsigned>(device_cus());
+  if (persist) {
+    k_zero_u32<<<1, 64, 0, st>>>(reinterpret_cast<unsigned*>(wa.queue), 4);
+    check(hipGetLastError(), "queue clear launch");
+  }
   if (ev) check(hipEventRecord(ev[8], st), "event record");
   if (LC_ROW_HALF != 0 && row_mode && wa.hbox == nullptr) {
     for (int kc = LC_ROW_HALF_KC; kc <= 32; ++kc) {
@@ -3553,13 +3601,20 @@ void launch_walk(const Sizes& z, const Bufs& B, const WalkArgs& wa,
     }
   }
 #define LC_WALK(KC)                                                             \
-  if (any_fast[KC] && row_mode) {                                               \
+  if (any_fast[KC] && row_mode && persist) {                                    \
+    constexpr int qi = KC == 4 ? 0 : KC == 8 ? 1 : KC == 16 ? 2 : 3;            \
+    const unsigned pb = std::min(row_blocks, cus * LC_ROW_BLOCKS(KC));          \
+    pa.queue = wa.queue + qi;                                                   \
+    if (wa.depth) k_walk_rows<KC, false, true, true><<<pb, 256, 0, st>>>(pa);   \
+    else k_walk_rows<KC, false, false, true><<<pb, 256, 0, st>>>(pa);           \
+    check(hipGetLastError(), "walk launch");                                    \
+  } else if (any_fast[KC] && row_mode) {                                        \
     if (wa.stats != nullptr) {                                                  \
-      if (wa.depth) k_walk_rows<KC, true, true><<<row_blocks, 256, 0, st>>>(wa);  \
-      else k_walk_rows<KC, true, false><<<row_blocks, 256, 0, st>>>(wa);         \
+      if (wa.depth) k_walk_rows<KC, true, true, false><<<row_blocks, 256, 0, st>>>(wa);  \
+      else k_walk_rows<KC, true, false, false><<<row_blocks, 256, 0, st>>>(wa);         \
     } else {                                                                    \
-      if (wa.depth) k_walk_rows<KC, false, true><<<row_blocks, 256, 0, st>>>(wa); \
-      else k_walk_rows<KC, false, false><<<row_blocks, 256, 0, st>>>(wa);        \
+      if (wa.depth) k_walk_rows<KC, false, true, false><<<row_blocks, 256, 0, st>>>(wa); \
+      else k_walk_rows<KC, false, false, false><<<row_blocks, 256, 0, st>>>(wa);        \
     }                                                                           \
     check(hipGetLastError(), "walk launch");                                    \
   } else if (any_fast[KC]) {                                                    \
@@ -3748,7 +3803,8 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
   // walk split: kRowWalk = rows everywhere; positive = a wave per dense
   // cell, one lane per source elsewhere
   WalkArgs wa = walk_args(z, B, dense_min_for(frames));
-  const bool row_mode = wa.dense_min == kRowWalk;
+  const bool row_mode = wa.dense_min == kRowWalk || wa.dense_min == kRowWalkPersist;
+  if (wa.dense_min == kRowWalkPersist) wa.queue = B.counters + 8;
   if (stats_on_ && !capturing_) {
     const long long nstat = kStatCount + 2 * (ntot / kWave + 1);
     grow(stats_, sizeof(unsigned long long) * static_cast<size_t>(nstat));
@@ -3842,7 +3898,7 @@ void Engine::run_dcell(const FrameSpec& spec, int n_frames, const double* cell,
   {
     size_t bytes[kNumBufs];
     buffer_bytes(z, bytes);
-    bytes[5] = sizeof(int) * 8;  // counters: the plan status rides in slot 4
+    bytes[5] = sizeof(int) * 12;  // counters: the plan status rides in slot 4
     Buf* own[kNumBufs] = {&frames_,  &fpt_,   &fcb_,      &count_,    &status_, &counters_,
                           &off_,     &tile_off_, &item_off_, &items_,   &cellslot_,
                           &tmp_key_, &cand_,  &scell_,    &byidx_,    &tbox_,   &tile_cell_,
@@ -4234,7 +4290,7 @@ void Engine::tile_labels(float* out, long long n) {
 }
 
 int Engine::autotune(const std::vector<FrameSpec>& frames, float* best_ms) {
-  static const int kCands[] = {kRowWalk, 1, 16, 32, 64, 128, kNeverDense};
+  static const int kCands[] = {kRowWalk, kRowWalkPersist, 1, 16, 32, 64, 128, kNeverDense};
   const int keep_override = dense_min_override_;
   const bool keep_timing = timing_;
   int best = kNeverDense;

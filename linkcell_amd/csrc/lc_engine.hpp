@@ -145,7 +145,7 @@ class Engine {
   // walk, the rest the lane-per-source walk.
   // An explicit value overrides the tuned table; 0 returns to it; -1
   // selects the row walk (one 16-lane row per 16-source tile).
-  void set_dense_min(int v) { dense_min_override_ = v < -1 ? 0 : v; }
+  void set_dense_min(int v) { dense_min_override_ = v < -2 ? 0 : v; }
   int dense_min_for(const std::vector<FrameSpec>& frames) const;
   // Walk schedule: 0 = plain; 1 = the learned predictor's k-th-neighbour
   // distance per source tile as the row walk's threshold cap; 3 = the

@@ -82,6 +82,27 @@ def test_c2_triclinic_full_size_sampled(ws):
     assert np.array_equal(reach[src], orr[src])
 
 
+@pytest.mark.parametrize("split", [-1, -2])
+def test_c1_c2_row_walk_launches_agree(ws, split):
+    """The row walk launched one wave per tile group and as one persistent
+    launch (waves taking groups from a device counter) give the oracle's
+    rows, reaches included."""
+    c1 = wl.uniform_c1()
+    g = np.load(GOLD / "c1_uniform_oracle.npz")
+    c = wl.triclinic_c2(n=65536)
+    ws.set_dense_min(split)
+    try:
+        nn, d2, reach = run_device(ws, c1["xyz"], c1["cell"], 12, None, 3.0, reach=True)
+        a = run_device(ws, c["xyz"], c["cell"], 16, c["mask"], 3.0)
+    finally:
+        ws.set_dense_min(0)
+    assert np.array_equal(nn, g["nn"])
+    assert np.array_equal(d2.view(np.uint64), g["d2_bits"])
+    assert np.array_equal(reach, g["reach"])
+    on, od = oracle(c["xyz"], c["cell"], 16, c["mask"], 3.0)
+    assert np.array_equal(a[0], on) and same_bits(a[1], od)
+
+
 def test_c2_deterministic_across_calls(ws):
     c = wl.triclinic_c2(n=65536)
     a = run_device(ws, c["xyz"], c["cell"], 16, c["mask"], 3.0)
