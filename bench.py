@@ -54,17 +54,19 @@ def workload(name):
     raise SystemExit(f"unknown config {name}")
 
 
-def fold_cells(xyz, cell, hint):
-    """Home cell ids of every point (numpy restatement of the fold and
-    bin; used only to count algorithmic bytes)."""
-    from oracle import lcoracle as orc
-    c12 = orc.cell12_of(cell)
-    oc = orc.Cell(c12[0:3], c12[3:6], c12[6:9], c12[9:12])
-    plan = oc.plan(hint)
-    H = c12[:9].reshape(3, 3)
-    o = c12[9:]
-    if oc.is_ortho:
-        s = (xyz - o) / np.array(oc.widths)
+def fold_cells(ws, xyz, c12, hint):
+    """Home cell ids of every point, used only to count algorithmic bytes:
+    the library's own plan of the cell (widths, orthorhombic flag, mesh,
+    lc_gpu_dcell_plan) and a numpy fold and bin with it."""
+    import torch
+    dc = torch.tensor(c12, dtype=torch.float64, device="cuda:0")
+    plan = ws.dcell_plan(dc.data_ptr(), 12, hint)
+    if plan["status"] != 0:
+        raise SystemExit(f"bench: the device could not plan the cell (status {plan['status']})")
+    H = np.asarray(plan["h"]).reshape(3, 3)
+    o = np.asarray(plan["origin"])
+    if plan["ortho"]:
+        s = (xyz - o) / np.asarray(plan["widths"])
     else:
         s = (xyz - o) @ np.linalg.inv(H)
     s = s - np.floor(s)
@@ -384,7 +386,7 @@ def main():
     reach = rr.cpu().numpy()
     ms_plain, kern_plain, work_plain = leg("plain")
     n_act = int((reach > 0).sum()) if mask is None else int(mask.sum())
-    cid, plan = fold_cells(xyz, cell, hint)
+    cid, plan = fold_cells(ws, xyz, c12, hint)
     bwalk, _ = walk_bytes(cid, plan, reach, k)
     split = args.dense_min if args.dense_min is not None else -1
     kc = 4 if k <= 4 else 8 if k <= 8 else 16 if k <= 16 else 32 if k <= 32 else 0
