@@ -5,7 +5,10 @@ FP64 points and k = 16 (hint 3.0), inputs resident in HBM. One step is
 one full search (fold, count, scan, scatter, cell sort, walk, packed
 outputs) through the lc_gpu_* C waist. Prints one JSON line.
 
-The plain walk is timed first; then the learned scheduler is trained on 32
+The library's autotuner first picks the walk launch for the frame's size
+bucket (the row walk launched per tile group or persistent; --no-autotune
+keeps the default row walk). The plain walk is timed first; then the
+learned scheduler is trained on 32
 further frames of the same mixture (labels from the plain walk, training on
 the device), and the learned walk is timed, checked bitwise against the
 plain walk's outputs. `value` is the learned walk's throughput; the plain
@@ -292,6 +295,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c3")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="skip the library's autotuner (which picks the walk launch "
+                         "for the frame's size bucket) and use the default row walk")
     ap.add_argument("--mode", type=int, default=0)
     ap.add_argument("--dense-min", type=int, default=None,
                     help="walk split: -1 row walk, else the home-cell occupancy at "
@@ -341,6 +347,14 @@ def main():
         ws.set_dense_min(args.dense_min)
     mptr = dm.data_ptr() if dm is not None else 0
     torch.cuda.synchronize()
+    tuned = None
+    if args.autotune and args.dense_min is None and args.mode == 0:
+        # the library's autotuner picks the walk launch for this frame's
+        # size bucket (lc_gpu_autotune); later searches use its choice
+        rec = dict(xyz=dx.data_ptr(), n=n, cell=c12, k=k, cell_hint=hint, mask=mptr,
+                   out_nn=nn.data_ptr(), out_d2=d2.data_ptr())
+        tuned, tune_ms = ws.autotune([rec])
+        log(f"bench: tuned split {tuned} ({tune_ms:.3f} ms)")
 
     def step(wait=False, reach=False):
         ws.knearest_ptr(dx.data_ptr(), n, 1, c12, k, nn.data_ptr(), d2.data_ptr(),
@@ -388,7 +402,8 @@ def main():
     n_act = int((reach > 0).sum()) if mask is None else int(mask.sum())
     cid, plan = fold_cells(ws, xyz, c12, hint)
     bwalk, _ = walk_bytes(cid, plan, reach, k)
-    split = args.dense_min if args.dense_min is not None else -1
+    split = args.dense_min if args.dense_min is not None else (tuned if tuned is not None else -1)
+    row_split = split in (-1, -2)
     kc = 4 if k <= 4 else 8 if k <= 8 else 16 if k <= 16 else 32 if k <= 32 else 0
 
     def roof(kern):
@@ -414,7 +429,7 @@ def main():
 
     learned = None
     learned_ok = (args.schedule == "learned" and args.config in TRAIN_FRAMES and kc != 0
-                  and split == -1 and args.mode == 0)
+                  and row_split and args.mode == 0)
     train_frames = (args.train_frames if args.train_frames is not None
                     else TRAIN_FRAMES.get(args.config, 0))
     if learned_ok:
@@ -486,14 +501,15 @@ def main():
         ms_step, kern, work, sched = ms_plain, kern_plain, work_plain, "plain"
         if args.mode == 1 or kc == 0:
             walk_kernel = "k_walk_exact"
-        elif split == -1:
-            walk_kernel = f"k_walk_rows<{kc}> (+ k_walk_exact for deferred sources)"
+        elif row_split:
+            walk_kernel = (f"k_walk_rows<{kc}>{' (persistent)' if split == -2 else ''} "
+                           "(+ k_walk_exact for deferred sources)")
         else:
             walk_kernel = (f"k_walk_dense<{kc}> + k_walk_sparse<{kc}> "
                            "(+ k_walk_exact for deferred sources)")
     t_walk, achieved = roof(kern)
     traffic, traffic_src = (pmc_traffic(cfg["name"], sched)
-                            if split == -1 and args.mode == 0 else (None, None))
+                            if row_split and args.mode == 0 else (None, None))
     value = n_act / (ms_step * 1e-3)
     out = {
         "metric": METRIC,
@@ -510,7 +526,8 @@ def main():
         "data": "synthetic (seeded generator, see linkcell_amd/workloads.py)",
         "config": {"workload": cfg["name"], "n": n, "k": k, "cell_hint": hint,
                    "mesh": [plan["nx"], plan["ny"], plan["nz"]], "schedule": sched,
-                   "mode": "auto" if args.mode == 0 else "exact-only"},
+                   "mode": "auto" if args.mode == 0 else "exact-only",
+                   "split": split, "autotuned": tuned is not None},
         "walk_ms": t_walk * 1e3,
         "walk_qps": n_act / t_walk,
         "kernels_ms": kern,
