@@ -980,6 +980,127 @@ __device__ __forceinline__ void merge8_regs(Top<KC>& h, long long (&b)[8],
   }
 }
 
+// Tagged heap keys (row walk, heaps of 8 slots and more). A key is the d2
+// bit pattern with its low kBits bits replaced by a slot tag, so every key
+// of a merge is distinct and the merge network moves keys alone: one
+// v_min_f64 and one v_max_f64 per compare-exchange, instead of those plus
+// the compare and two selects that carry an index along. Each lane keeps,
+// per slot, (index << kBits | the d2's own low bits) in LDS; after the
+// network every output slot gathers its entry by the tag it carries, and
+// the slots are tagged again in order. Two d2 that agree above the tag
+// bits (within 2^kBits ulps) may leave the network in tag order rather
+// than d2 order: inside the kept k the emit restores (d2, index) order,
+// and when the bucket of the k-th holds a dropped key too the source goes
+// to the exact-order walk, as for an exact boundary tie. A key set only
+// ever grows above the k-th by keys of its own bucket or larger, so a
+// swapped pair is always seen at that test.
+#ifndef LC_TAG_MERGE
+#define LC_TAG_MERGE 1
+#endif
+template <int KC>
+struct TagKey {
+  static constexpr int kBits = KC + 8 <= 32 ? 5 : 6;
+  static constexpr long long kMask = (1ll << kBits) - 1;
+  // an empty slot: the largest finite double, tag bits cleared (above any
+  // d2 of a frame the tagged walk takes, tag_frame_ok); an unused slot (k
+  // below the bucket size): -1.0, below any d2
+  static constexpr long long kEmpty = 0x7fefffffffffffffll & ~kMask;
+  static constexpr long long kUnused = static_cast<long long>(0xbff0000000000000ull);
+  // indices stored with the low bits: frames of fewer points than this
+  static constexpr long long kMaxN = 1ll << (32 - kBits);
+};
+
+template <int KC>
+__device__ __forceinline__ void tag_init(Top<KC>& h, int k) {
+#pragma unroll
+  for (int t = 0; t < KC; ++t) {
+    h.d[t] = (t < KC - k ? TagKey<KC>::kUnused : TagKey<KC>::kEmpty) | t;
+  }
+}
+
+// Whether the tagged row walk can take a frame: every index fits beside
+// the tag bits, and every d2 of the frame's walk (folded points and one
+// lattice shift per axis, each coordinate difference below 9 times the
+// largest cell entry) stays finite and below the empty key.
+__device__ __forceinline__ bool tag_frame_ok(const FrameDesc& d, long long max_n) {
+  double e = 0.0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) e = fmax(e, fabs(d.h[i]));
+#pragma unroll
+  for (int i = 0; i < 3; ++i) e = fmax(e, fabs(d.widths[i]));
+  return static_cast<long long>(d.n) < max_n && e < 1e150;
+}
+
+// merge8_regs on tagged keys: the odd-even sort of the 8 buffered keys
+// (tagged KC + t; pads carry the empty key), the element-wise min against
+// the top 8 slots, the bitonic clean. A key's tag names its entry's slot,
+// and entries stay where they are: a buffered key that displaces a heap key
+// takes that key's slot (reloc copies its entry there, while the clean
+// runs), so the kept keys' tags are always the slots 0 .. KC - 1. min_rej
+// takes every key a pair drops.
+// v_min_f64 / v_max_f64 of two keys, written out so that the compiler adds
+// no quieting of signalling NaNs before them (keys are never NaN). Measured
+// (walk, tagged keys): clustered 4M frame 3.86 against 3.93 ms (persistent
+// launch), FCC 8M frame (k = 32) 8.86 against 8.95 ms.
+#ifndef LC_TAG_ASM
+#define LC_TAG_ASM 1
+#endif
+__device__ __forceinline__ double raw_min(double a, double b) {
+#if LC_TAG_ASM
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+#else
+  return key_min(a, b);
+#endif
+}
+__device__ __forceinline__ double raw_max(double a, double b) {
+#if LC_TAG_ASM
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+#else
+  return key_max(a, b);
+#endif
+}
+
+template <int KC, typename Reloc>
+__device__ __forceinline__ void merge8_keys(long long (&hd)[KC], long long (&b)[8],
+                                            double& min_rej, Reloc&& reloc) {
+  constexpr long long M = TagKey<KC>::kMask;
+  auto cs = [](long long& x, long long& y) LC_INLINE {
+    const double a = as_f64(x), c = as_f64(y);
+    x = __double_as_longlong(raw_min(a, c));
+    y = __double_as_longlong(raw_max(a, c));
+  };
+  cs(b[0], b[1]), cs(b[2], b[3]), cs(b[4], b[5]), cs(b[6], b[7]);
+  cs(b[0], b[2]), cs(b[1], b[3]), cs(b[4], b[6]), cs(b[5], b[7]);
+  cs(b[1], b[2]), cs(b[5], b[6]);
+  cs(b[0], b[4]), cs(b[1], b[5]), cs(b[2], b[6]), cs(b[3], b[7]);
+  cs(b[2], b[4]), cs(b[3], b[5]);
+  cs(b[1], b[2]), cs(b[3], b[4]), cs(b[5], b[6]);
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const long long x = hd[KC - 8 + t], y = b[7 - t];
+    const double xf = as_f64(x), yf = as_f64(y);
+    const bool take = yf < xf;  // keys are distinct: never equal
+    min_rej = raw_min(min_rej, raw_max(xf, yf));
+    const int ty = static_cast<int>(y & M), tx = static_cast<int>(x & M);
+    hd[KC - 8 + t] = take ? ((y & ~M) | tx) : x;
+    // branch-free: when the heap key stays, the buffered entry is copied
+    // onto itself
+    reloc(ty, take ? tx : ty);
+  }
+#pragma unroll
+  for (int stride = KC >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+    for (int i = 0; i < KC; ++i) {
+      const int p = i ^ stride;
+      if (p > i) cs(hd[i], hd[p]);
+    }
+  }
+}
+
 template <int KC>
 __device__ __forceinline__ void merge8(Top<KC>& h, const long long* bd,
                                        const int* bj, int n, double& min_rej) {
@@ -1083,6 +1204,50 @@ __device__ __forceinline__ bool top_emit(Top<KC>& h, int k, double min_rej,
   }
   return true;
 }
+
+// Finish a source of the tagged walk: exact d2 and indices from the keys
+// and the lane's LDS entries (pk[t]: the entry slot t's tag names), the boundary test on the
+// tag bucket of the k-th, (d2, index) order inside the kept k, then
+// top_emit. Returns false to defer the source.
+template <int KC>
+__device__ __forceinline__ bool tag_emit(const Top<KC>& h, const unsigned (&pk)[KC], int k,
+                                         double min_rej, int* out_nn, double* out_d2,
+                                         size_t row) {
+  constexpr long long M = TagKey<KC>::kMask;
+  if ((__double_as_longlong(min_rej) & ~M) == (h.d[KC - 1] & ~M)) return false;
+  Top<KC> e;
+#pragma unroll
+  for (int t = 0; t < KC; ++t) {
+    const long long key = h.d[t] & ~M;
+    const bool unused = key < 0;
+    const bool empty = key == TagKey<KC>::kEmpty;
+    e.d[t] = unused ? kUnusedBits : (empty ? kInfBits : (key | (pk[t] & M)));
+    e.j[t] = (unused || empty) ? kEmptyJ : static_cast<int>(pk[t] >> TagKey<KC>::kBits);
+    // pk[t]: the entry of slot t's tag
+  }
+  bool dis = false;
+#pragma unroll
+  for (int t = 0; t + 1 < KC; ++t) dis |= (t >= KC - k) && e.d[t] > e.d[t + 1];
+  if (dis) {
+    // keys of one tag bucket in tag order: sort the kept k by d2 (the
+    // bit patterns of d2 >= 0 order as integers)
+#pragma unroll 1
+    for (int pass = 0; pass < KC; ++pass) {
+#pragma unroll
+      for (int t = 0; t + 1 < KC; ++t) {
+        const bool sw = (t >= KC - k) && e.d[t] > e.d[t + 1];
+        const long long a = e.d[t];
+        const int aj = e.j[t];
+        e.d[t] = sw ? e.d[t + 1] : a;
+        e.j[t] = sw ? e.j[t + 1] : aj;
+        e.d[t + 1] = sw ? a : e.d[t + 1];
+        e.j[t + 1] = sw ? aj : e.j[t + 1];
+      }
+    }
+  }
+  return top_emit<KC>(e, k, -__builtin_inf(), out_nn, out_d2, row);
+}
+
 
 __device__ __forceinline__ void defer_source(int s, int* defer_list,
                                              int* defer_count) {
@@ -1838,6 +2003,27 @@ struct RowOcc {
 #else
 #define LC_ROW_BLOCKS(KC) RowOcc<KC>::kMinBlocks
 #endif
+// Per-wave candidate buffers of the row walk (4 waves, 8 buffered candidates
+// per lane): 16-byte records (d2 bits, index) for the plain heap; for the
+// tagged heap, the d2 bits and each lane's slot entries, the KC heap slots
+// (index << tag bits | the d2's low bits) and then the 8 buffer slots
+// (indices until a merge packs them). Measured in LDS per block: 32 KB,
+// or 16 KB + (KC + 8) KB for the tagged heap (40 KB at KC = 16, which
+// still leaves 3 blocks per CU).
+template <int KC, bool kTag>
+struct RowBufs;
+template <int KC>
+struct RowBufs<KC, false> {
+  int4 buf[4][8][kWave];
+};
+template <int KC>
+struct RowBufs<KC, true> {
+  // per wave, slots of one dword per lane: the KC + 8 entries, then the low
+  // and the high halves of the 8 buffered d2 (one slot stride for all, so
+  // an append's three stores share one address)
+  unsigned dw[4][KC + 24][kWave];
+};
+
 // kDepth: the depth schedule (rows list shells 1 .. predicted depth in one
 // pass); without it every row of a wave is in the same shell. kPersist: the
 // grid fills the GPU once and every wave takes groups of four tiles from a
@@ -1860,9 +2046,14 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   constexpr int LC = kFill - 1 + (kHalf ? 32 : 16);
   static_assert(kFill >= 1 && LC <= 64, "tile list capacity");
   constexpr int NK = (LC + 15) / 16;  // list keys per lane when selecting
+  // tagged heap keys for heaps of 8 slots and more
+  constexpr bool kTag = LC_TAG_MERGE != 0 && KC >= 8;
+  constexpr long long TM = TagKey<KC>::kMask;
   // buffered candidates: one 16-byte record (d2 bits, index) per slot and
-  // lane, so an append is one address and one ds_write_b128
-  __shared__ int4 s_buf[W][BC][kWave];
+  // lane, so an append is one address and one ds_write_b128; or, tagged,
+  // the d2 bits and the index in the lane's slot entries
+  static_assert(W == 4 && BC == 8, "RowBufs shape");
+  __shared__ RowBufs<KC, kTag> s_rb;
   // tile list entries: (first slot, wrap code | members << 5), one 8-byte
   // record so a fetch is one LDS read
   __shared__ int2 s_list[W][4][LC];
@@ -1918,6 +2109,13 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     const int s = o0 + (valid ? srel : 0);
     const bool frame_ok = (A.fflags[f] & kFrameNonFinite) == 0 && d.r_safe >= 1;
     if (!frame_ok) {
+      if (valid) defer_source(s, A.defer_list, A.defer_count);
+      continue;
+    }
+    if (kTag && !tag_frame_ok(d, TagKey<KC>::kMaxN)) {
+      // indices or distances beyond what the tagged keys hold (frames of
+      // 2^27 points and more, 2^26 for the 32-slot heap, or cells of 1e150
+      // and more): the exact-order walk takes them
       if (valid) defer_source(s, A.defer_list, A.defer_count);
       continue;
     }
@@ -1988,7 +2186,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     };
 
     Top<KC> h;
-    top_init<KC>(h, k);
+    if constexpr (kTag) tag_init<KC>(h, k); else top_init<KC>(h, k);
     double min_rej = inf;
     // threshold cap from the predicted k-th distance of the home cell (+inf
     // without a prediction). Candidates beyond it are never kept; the shell-
@@ -2036,14 +2234,33 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       if (kStats) st_flush += 1;
       if (nbuf > 0) {
         if (kStats) st_merges += 1;
-        if constexpr (KC >= 8) {
+        if constexpr (kTag) {
+          // buffered keys tagged KC + t, their entries packed; pads carry
+          // the empty key
+          long long b[8];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) {
+            const bool v = t < nbuf;
+            const unsigned elo = s_rb.dw[wv][KC + 8 + t][lane];
+            const unsigned ehi = s_rb.dw[wv][KC + 16 + t][lane];
+            const long long e = (static_cast<long long>(ehi) << 32) | elo;
+            const unsigned jx = s_rb.dw[wv][KC + t][lane];
+            // (a pad's entry is packed too: no key names it)
+            s_rb.dw[wv][KC + t][lane] =
+                (jx << TagKey<KC>::kBits) | (elo & static_cast<unsigned>(TM));
+            b[t] = (v ? (e & ~TM) : TagKey<KC>::kEmpty) | (KC + t);
+          }
+          merge8_keys<KC>(h.d, b, min_rej, [&](int from, int to) LC_INLINE {
+            s_rb.dw[wv][to][lane] = s_rb.dw[wv][from][lane];
+          });
+        } else if constexpr (KC >= 8) {
           // the buffer is read straight from the shared arrays (LDS loads)
           long long b[8];
           int bx[8];
 #pragma unroll
           for (int t = 0; t < 8; ++t) {
             const bool v = t < nbuf;
-            const int4 e = s_buf[wv][t][lane];
+            const int4 e = s_rb.buf[wv][t][lane];
             b[t] = v ? (static_cast<long long>(e.y) << 32) |
                            static_cast<long long>(static_cast<unsigned>(e.x))
                      : kInfBits;
@@ -2055,7 +2272,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
           int bx[8];
 #pragma unroll
           for (int t = 0; t < 8; ++t) {
-            const int4 e = s_buf[wv][t][lane];
+            const int4 e = s_rb.buf[wv][t][lane];
             b[t] = (static_cast<long long>(e.y) << 32) |
                    static_cast<long long>(static_cast<unsigned>(e.x));
             bx[t] = e.z;
@@ -2063,7 +2280,11 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
           insert_merge_regs<KC>(h, b, bx, nbuf, min_rej);
         }
         nbuf = 0;
-        thr = live ? fmin(as_f64(h.d[KC - 1]), thr_cap) : -inf;
+        // tagged: the k-th key with its tag bits set bounds the k-th d2
+        // from above (the accept test then takes a few more candidates of
+        // the k-th's own tag bucket, never fewer)
+        const long long wk = kTag ? (h.d[KC - 1] | TM) : h.d[KC - 1];
+        thr = live ? fmin(as_f64(wk), thr_cap) : -inf;
       }
       rthr = row_max16(live ? thr : -inf);
       if (kStats) cyc_merge += __builtin_readcyclecounter() - t0;
@@ -2107,7 +2328,14 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
         // indexed on the shared arrays themselves, so the stores stay LDS
         // stores whatever the register allocator does with pointers
         const long long db = __double_as_longlong(d2);
-        s_buf[wv][nbuf][lane] = make_int4(static_cast<int>(db), static_cast<int>(db >> 32), qj, 0);
+        if constexpr (kTag) {
+          s_rb.dw[wv][KC + 8 + nbuf][lane] = static_cast<unsigned>(db);
+          s_rb.dw[wv][KC + 16 + nbuf][lane] = static_cast<unsigned>(db >> 32);
+          s_rb.dw[wv][KC + nbuf][lane] = static_cast<unsigned>(qj);
+        } else {
+          s_rb.buf[wv][nbuf][lane] =
+              make_int4(static_cast<int>(db), static_cast<int>(db >> 32), qj, 0);
+        }
         nbuf += 1;
         if (kStats) {
           st_acc += 1;
@@ -2645,11 +2873,22 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       if (__ballot(nbuf > 0)) flush();
       // stop test, src/knearest.rs:466-471
       const double bound = static_cast<double>(r) * d.cell_min;
-      if (live && h.j[KC - 1] != kEmptyJ) {
+      // the k-th d2 exactly (tagged: its key with the d2's own low bits)
+      // and whether the heap is full
+      long long kth = h.d[KC - 1];
+      bool full = false;
+      if constexpr (kTag) {
+        full = (kth & ~TM) != TagKey<KC>::kEmpty;
+        kth = (kth & ~TM) |
+              (s_rb.dw[wv][static_cast<int>(kth & TM)][lane] & static_cast<unsigned>(TM));
+      } else {
+        full = h.j[KC - 1] != kEmptyJ;
+      }
+      if (live && full) {
         // a full heap holds only d2 <= thr_cap, and everything skipped or
         // rejected lies beyond it: the heap is the exact top-k of the shells
         // walked so far
-        if (as_f64(h.d[KC - 1]) <= bound * bound) {
+        if (as_f64(kth) <= bound * bound) {
           live = false;
           done_reach = r;
         }
@@ -2670,7 +2909,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     if (row_fail && li == 0 && A.retry_count != nullptr) atomicAdd(A.retry_count, 1);
     // the failed rows start over without the cap; every other row is done
     if (row_fail) {
-      top_init<KC>(h, k);
+      if constexpr (kTag) tag_init<KC>(h, k); else top_init<KC>(h, k);
       min_rej = inf;
       thr_cap = inf;
       rp = 1;
@@ -2711,8 +2950,18 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     }
     if (!valid) continue;
     const size_t row = static_cast<size_t>(me) * k;
-    if (done_reach == 0 ||
-        !top_emit<KC>(h, k, min_rej, d.out_nn, d.out_d2, row)) {
+    bool emitted = false;
+    if (done_reach != 0) {
+      if constexpr (kTag) {
+        unsigned pk[KC];
+#pragma unroll
+        for (int t = 0; t < KC; ++t) pk[t] = s_rb.dw[wv][static_cast<int>(h.d[t] & TM)][lane];
+        emitted = tag_emit<KC>(h, pk, k, min_rej, d.out_nn, d.out_d2, row);
+      } else {
+        emitted = top_emit<KC>(h, k, min_rej, d.out_nn, d.out_d2, row);
+      }
+    }
+    if (!emitted) {
       defer_source(s, A.defer_list, A.defer_count);
       continue;
     }
