@@ -1863,12 +1863,16 @@ __device__ __forceinline__ int row_bcast(int v) {
 //    (ls32 <= ls64 (1 + 2^-21)), and the threshold is the float at or above
 //    it times (1 + 2^-19), so ls32 above it puts ls64 above the threshold;
 //  * the list key is the smallest ls32 of the sources that keep the tile,
-//    times (1 - 2^-20): at or below the FP64 bound it replaces.
-// Flushing FP32 subnormals to zero only lowers a gap or a sum, and a
-// coordinate beyond float range makes a NaN or zero gap, which keeps the
-// tile. Measured (walk): clustered 4M frame 3.65 against 3.85 ms
-// (persistent launch), FCC 8M frame (k = 32) 8.56 against 8.87 ms, with
-// 0.1% more tiles swept.
+//    at most FLT_MAX, times (1 - 2^-20): at or below the FP64 bound it
+//    replaces (a sum that overflows to +inf in FP32 has an FP64 bound of
+//    FLT_MAX or more; the key must not be +inf, which the drain would take
+//    for a tile beyond every threshold).
+// Coordinates beyond float range are clamped to +-FLT_MAX after the
+// conversion: the clamp is monotone and shrinks every difference, so gaps
+// only fall, and no infinity - infinity NaN can arise. Flushing FP32
+// subnormals to zero only lowers a gap or a sum. Measured (walk): clustered
+// 4M frame 3.65 against 3.85 ms (persistent launch), FCC 8M frame (k = 32)
+// 8.56 against 8.87 ms, with 0.1% more tiles swept.
 #ifndef LC_LIST32
 #define LC_LIST32 1
 #endif
@@ -1876,6 +1880,14 @@ template <int I>
 __device__ __forceinline__ float row_bcast_f32(float v) {
   return __builtin_bit_cast(
       float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x150 + I, 0xf, 0xf, true));
+}
+// a coordinate as a float, clamped to the finite range (v_med3_f32)
+__device__ __forceinline__ float coord32(double x) {
+  return __builtin_amdgcn_fmed3f(static_cast<float>(x), -3.40282347e38f, 3.40282347e38f);
+}
+// a bound as the list key's double: at most FLT_MAX, scaled down
+__device__ __forceinline__ double key_of32(float b) {
+  return fmin(static_cast<double>(b), 3.4028234663852886e38) * (1.0 - 0x1p-20);
 }
 // the float at or above t (t >= 0 or -inf), times (1 + 2^-19)
 __device__ __forceinline__ float thr_up32(double t) {
@@ -2186,8 +2198,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     const int me = static_cast<int>(mc.id);
     const double inf = as_f64(kInfBits);
     // the listing's FP32 copies (see LC_LIST32)
-    const float px32 = static_cast<float>(px), py32 = static_cast<float>(py),
-                pz32 = static_cast<float>(pz);
+    const float px32 = coord32(px), py32 = coord32(py), pz32 = coord32(pz);
     const float eps32 = list_eps32(d);
     // bounding box of the row's sources, kept in LDS (read only while
     // listing tiles)
@@ -2810,12 +2821,12 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
                   bool any_a = false, any_b = false;
 #if LC_LIST32
                   const float tq32 = thr_up32(tq);
-                  const float fa[6] = {static_cast<float>(ax0) - eps32, static_cast<float>(ax1) + eps32,
-                                       static_cast<float>(ay0) - eps32, static_cast<float>(ay1) + eps32,
-                                       static_cast<float>(az0) - eps32, static_cast<float>(az1) + eps32};
-                  const float fb[6] = {static_cast<float>(cx0) - eps32, static_cast<float>(cx1) + eps32,
-                                       static_cast<float>(cy0) - eps32, static_cast<float>(cy1) + eps32,
-                                       static_cast<float>(cz0) - eps32, static_cast<float>(cz1) + eps32};
+                  const float fa[6] = {coord32(ax0) - eps32, coord32(ax1) + eps32,
+                                       coord32(ay0) - eps32, coord32(ay1) + eps32,
+                                       coord32(az0) - eps32, coord32(az1) + eps32};
+                  const float fb[6] = {coord32(cx0) - eps32, coord32(cx1) + eps32,
+                                       coord32(cy0) - eps32, coord32(cy1) + eps32,
+                                       coord32(cz0) - eps32, coord32(cz1) + eps32};
                   float lb_a32 = __builtin_inff(), lb_b32 = __builtin_inff();
                   auto probe_src = [&](auto I) LC_INLINE {
                     constexpr int ii = decltype(I)::value;
@@ -2884,8 +2895,8 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
                   probe_src(std::integral_constant<int, 14>{});
                   probe_src(std::integral_constant<int, 15>{});
 #if LC_LIST32
-                  lb_a = static_cast<double>(lb_a32) * (1.0 - 0x1p-20);
-                  lb_b = static_cast<double>(lb_b32) * (1.0 - 0x1p-20);
+                  lb_a = key_of32(lb_a32);
+                  lb_b = key_of32(lb_b32);
 #endif
                   // a tile of 8 or fewer has an empty second half, whose
                   // (+inf, -inf) box passes an infinite threshold
@@ -2917,12 +2928,9 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
                   double lbs = inf;  // smallest per-source bound (a tighter key)
 #if LC_LIST32
                   const float tq32 = thr_up32(tq);
-                  const float fx0 = static_cast<float>(bx0) - eps32;
-                  const float fx1 = static_cast<float>(bx1) + eps32;
-                  const float fy0 = static_cast<float>(by0) - eps32;
-                  const float fy1 = static_cast<float>(by1) + eps32;
-                  const float fz0 = static_cast<float>(bz0) - eps32;
-                  const float fz1 = static_cast<float>(bz1) + eps32;
+                  const float fx0 = coord32(bx0) - eps32, fx1 = coord32(bx1) + eps32;
+                  const float fy0 = coord32(by0) - eps32, fy1 = coord32(by1) + eps32;
+                  const float fz0 = coord32(bz0) - eps32, fz1 = coord32(bz1) + eps32;
                   float lbs32 = __builtin_inff();
                   auto probe_src = [&](auto I) LC_INLINE {
                     constexpr int ii = decltype(I)::value;
@@ -2971,7 +2979,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
                   probe_src(std::integral_constant<int, 14>{});
                   probe_src(std::integral_constant<int, 15>{});
 #if LC_LIST32
-                  lbs = static_cast<double>(lbs32) * (1.0 - 0x1p-20);
+                  lbs = key_of32(lbs32);
 #endif
                   need = need && any;
                   if (LC_ROW_SELECT) lbt = key_max(lbt, lbs);
