@@ -2,9 +2,10 @@
 handle (orthorhombic and triclinic boxes down to the size of a cell, uniform
 and clustered points, lattices with exact distance ties, coincident points,
 points several images outside the box, masks, several frames, k from 1 to
-above the point count) through every walk and entry point, each against the
-oracle (the reference's KHeap walk, src/knearest.rs:351-486) bit for bit,
-per-source reach included."""
+above the point count, coordinate scales from 1e-3 to 1e45, beyond float
+range) through every walk and entry point, each against the oracle (the
+reference's KHeap walk, src/knearest.rs:351-486) bit for bit, per-source
+reach included."""
 import numpy as np
 import pytest
 
@@ -52,6 +53,13 @@ def make_case(seed):
     frames = int(rng.choice([1, 1, 2]))
     if frames > 1:
         xyz = np.concatenate([xyz, xyz + rng.normal(0, 0.05, xyz.shape)])
+    scale = float(rng.choice([1.0, 1.0, 1.0, 1e-3, 1e30, 1e45]))
+    if scale != 1.0:
+        # the hint scales with the box, so the mesh stays within the
+        # reference's cell limit
+        xyz = xyz * scale
+        cell = cell * scale
+        hint = max(hint, 1.0) * scale
     return dict(xyz=xyz, cell=cell, k=k, mask=mask, hint=hint, frames=frames, n=n)
 
 
