@@ -2478,14 +2478,18 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     if (kStats && attempt == 0) cyc_init = to0 - cyc0;
     // the next tile's candidates are loaded before the own tile is swept, so
     // the load overlaps that sweep
-    double nbx = nan, nby = nan, nbz = nan;
-    int nbj = -1;
-    if (has_tile && tl_nb >= 0 && tl_nb * kTile + li < cnt) {
-      const Cand q = A.cand[o0 + tl_nb * kTile + li];
-      nbx = q.x;
-      nby = q.y;
-      nbz = q.z;
-      nbj = static_cast<int>(q.id);
+    // (loaded unconditionally, from the source's own slot when the lane
+    // has no candidate, so that the loads sit in one block and the waits
+    // before each sweep count exactly the loads it needs)
+    double nbx, nby, nbz;
+    int nbj;
+    {
+      const bool ok = has_tile && tl_nb >= 0 && tl_nb * kTile + li < cnt;
+      const Cand q = A.cand[ok ? o0 + tl_nb * kTile + li : s];
+      nbx = ok ? q.x : nan;
+      nby = ok ? q.y : nan;
+      nbz = ok ? q.z : nan;
+      nbj = ok ? static_cast<int>(q.id) : -1;
     }
     cx = valid ? px : nan;
     cy = valid ? py : nan;
@@ -2498,15 +2502,12 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     cj = nbj;
     if (LC_OPEN3) {
       // the third tile's loads overlap the second tile's sweep
-      nbx = nby = nbz = nan;
-      nbj = -1;
-      if (has_tile && tl_nb2 >= 0 && tl_nb2 * kTile + li < cnt) {
-        const Cand q = A.cand[o0 + tl_nb2 * kTile + li];
-        nbx = q.x;
-        nby = q.y;
-        nbz = q.z;
-        nbj = static_cast<int>(q.id);
-      }
+      const bool ok = has_tile && tl_nb2 >= 0 && tl_nb2 * kTile + li < cnt;
+      const Cand q = A.cand[ok ? o0 + tl_nb2 * kTile + li : s];
+      nbx = ok ? q.x : nan;
+      nby = ok ? q.y : nan;
+      nbz = ok ? q.z : nan;
+      nbj = ok ? static_cast<int>(q.id) : -1;
     }
     if (__ballot(tl_nb >= 0 && has_tile)) sweep(std::false_type{});
     if (LC_OPEN3) {
