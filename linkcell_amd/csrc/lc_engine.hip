@@ -3105,6 +3105,50 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
   } while (kPersist);
 }
 
+#ifndef LC_CAP_CELL_MAX
+#define LC_CAP_CELL_MAX 16
+#endif
+// Tiles whose learned cap the walk uses (LC_CAP_CELL_MAX): the one tile of
+// each cell of 1 .. cmax members goes to the list, to which inference is
+// restricted; every other tile gets no cap (+inf). Caps help where the
+// opening cannot fill the heap (home cells of at most 16 points, whose
+// sources take every candidate until their first merge) and barely
+// elsewhere.
+// One atomic per 1024-thread block on the list's count (the list's order
+// does not matter): atomics on one address serialize, and one per wave took
+// 37 us on the clustered 4M frame's 192,000 cells.
+constexpr int kCapThreads = 1024;
+__global__ __launch_bounds__(kCapThreads) void k_cap_tiles(
+    const int* __restrict__ count, const int* __restrict__ tile_off, long long ncells,
+    int cmax, float* __restrict__ caps, int* __restrict__ list, int* __restrict__ list_n) {
+  __shared__ int s_off[kCapThreads / kWave];
+  __shared__ int s_base;
+  const long long g = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+  const bool in = g < ncells;
+  const int c = in ? count[g] : 0;
+  const int t0 = in ? tile_off[g] : 0, t1 = in ? tile_off[g + 1] : 0;
+  const bool take = in && c >= 1 && c <= cmax && t1 - t0 == 1;
+  const unsigned long long m = __ballot(take);
+  if (lane == 0) s_off[wv] = __popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < kCapThreads / kWave; ++w) {
+      const int v = s_off[w];
+      s_off[w] = tot;
+      tot += v;
+    }
+    s_base = tot > 0 ? atomicAdd(list_n, tot) : 0;
+  }
+  __syncthreads();
+  if (take) list[s_base + s_off[wv] + __popcll(m & ((1ull << lane) - 1ull))] = t0;
+  if (in && !take) {
+    for (int t = t0; t < t1; ++t) caps[t] = __builtin_inff();
+  }
+}
+
 // Queue every active source of the frames that have no register-heap
 // bucket (k above the largest bucket, or the exact-only mode).
 __global__ __launch_bounds__(256) void k_route_exact(
@@ -3692,7 +3736,7 @@ void Engine::free_all() {
                 &scell_,    &byidx_,  &tbox_,   &cbox_,
                 &defer_,    &heap_d2_,  &heap_idx_, &flags_, &stats_,
                 &sched_,    &sched_cnt_, &tile_cell_,
-                &pred_tile_, &labels_};
+                &pred_tile_, &labels_, &cap_list_};
   for (Buf* b : all) {
     if (b->p != nullptr) (void)hipFree(b->p);
     b->p = nullptr;
@@ -3833,7 +3877,7 @@ void buffer_bytes(const Sizes& z, size_t (&b)[kNumBufs]) {
       sizeof(long long) * (F + 1),                   // fcb
       sizeof(int) * C,                               // count
       sizeof(unsigned long long) * 2 * static_cast<size_t>(z.tiles),  // status
-      sizeof(int) * 12,                              // counters (8..11: walk queues)
+      sizeof(int) * 12,                              // counters (6: device-cell plan status, 8..11: walk queues)
       sizeof(int) * (C + 1),                         // off
       sizeof(int) * (C + 1),                         // tile_off
       sizeof(int) * (C + 1),                         // item_off
@@ -3881,7 +3925,8 @@ void launch_prep(const Sizes& z, const Bufs& B, hipStream_t st, hipEvent_t* ev) 
                                                     ctot);
   k_zero_u32<<<blocks_for(tiles * 4, 256), 256, 0, st>>>(
       reinterpret_cast<unsigned*>(B.status), tiles * 4);
-  k_zero_u32<<<1, 64, 0, st>>>(reinterpret_cast<unsigned*>(B.counters), 4);
+  // counters 0-5: scans, deferred sources, retried rows, learned-cap tiles
+  k_zero_u32<<<1, 64, 0, st>>>(reinterpret_cast<unsigned*>(B.counters), 6);
   k_zero_u32<<<blocks_for(F, 256), 256, 0, st>>>(reinterpret_cast<unsigned*>(B.flags), F);
 
   k_fold_count<<<blocks_for(ntot, 256), 256, 0, st>>>(B.frames, B.fpt, F, ntot, B.count,
@@ -4204,7 +4249,21 @@ void Engine::run(const std::vector<FrameSpec>& frames, bool wait) {
     // learned caps of every source tile, from the counts and tile boxes
     // just built
     grow(pred_tile_, sizeof(float) * static_cast<size_t>(tcap));
-    pred_->predict_caps(tile_src(ctot, F), tcap, as<float>(pred_tile_.p), st);
+    if (sched_mode_ == 1 && LC_CAP_CELL_MAX > 0) {
+      // caps only for the tiles of sparse home cells, inferred for those
+      // alone
+      grow(cap_list_, sizeof(int) * static_cast<size_t>(tcap));
+      int* const lst = as<int>(cap_list_.p);
+      int* const lcnt = B.counters + 5;  // cleared with the preparation's counters
+      k_cap_tiles<<<blocks_for(ctot, kCapThreads), kCapThreads, 0, st>>>(B.count, B.tile_off, ctot,
+                                                         LC_CAP_CELL_MAX,
+                                                         as<float>(pred_tile_.p), lst, lcnt);
+      check(hipGetLastError(), "cap tiles launch");
+      pred_->predict_caps(tile_src(ctot, F), tcap, as<float>(pred_tile_.p), st, nullptr, lst,
+                          lcnt);
+    } else {
+      pred_->predict_caps(tile_src(ctot, F), tcap, as<float>(pred_tile_.p), st);
+    }
     wa.pred_tile = as<float>(pred_tile_.p);
     wa.pred_tile_n = tcap;
     if (sched_mode_ == 3) {
@@ -4283,7 +4342,7 @@ void Engine::run_dcell(const FrameSpec& spec, int n_frames, const double* cell,
   {
     size_t bytes[kNumBufs];
     buffer_bytes(z, bytes);
-    bytes[5] = sizeof(int) * 12;  // counters: the plan status rides in slot 4
+    bytes[5] = sizeof(int) * 12;  // counters: the plan status rides in slot 6
     Buf* own[kNumBufs] = {&frames_,  &fpt_,   &fcb_,      &count_,    &status_, &counters_,
                           &off_,     &tile_off_, &item_off_, &items_,   &cellslot_,
                           &tmp_key_, &cand_,  &scell_,    &byidx_,    &tbox_,   &tile_cell_,
@@ -4308,7 +4367,7 @@ void Engine::run_dcell(const FrameSpec& spec, int n_frames, const double* cell,
   d.k = spec.k;
   d.kc = kc;
   k_plan_dcell<<<1, 64, 0, st>>>(cell, cell_n, hint, d, F, n, spec.k, dcell_cap_, B.frames,
-                                 B.fpt, B.fcb, B.counters + 4);
+                                 B.fpt, B.fcb, B.counters + 6);
   check(hipGetLastError(), "plan launch");
   const bool timing = timing_ && !capturing_;
   if (timing) {
@@ -4335,7 +4394,7 @@ void Engine::check_dcell_status() {
   if (!dcell_pending_) return;
   dcell_pending_ = false;
   int stv = 0;
-  check(hipMemcpy(&stv, as<int>(counters_.p) + 4, sizeof(int), hipMemcpyDeviceToHost),
+  check(hipMemcpy(&stv, as<int>(counters_.p) + 6, sizeof(int), hipMemcpyDeviceToHost),
         "DtoH plan status");
   if (stv == 1) throw std::runtime_error("bad cell");
   if (stv == 2) throw std::runtime_error("too many cells");

@@ -234,6 +234,7 @@ class Engine {
   long long last_ntot_ = 0, last_ctot_ = 0;
   std::unique_ptr<Predictor> pred_;
   Buf pred_tile_, labels_, tile_perm_;
+  Buf cap_list_;  // tiles the learned caps are inferred for, then their count
   TileSrc tile_src(long long ctot, int F);
   void* staging_ = nullptr;
   size_t staging_bytes_ = 0;

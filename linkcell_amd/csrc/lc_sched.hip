@@ -567,24 +567,33 @@ __global__ __launch_bounds__(256) void k_mlp_forward(
 }
 
 // Learned cap of every tile of a search: features straight into LDS, the
-// forward pass, out[T] = cell_min * 2^(y + margin).
+// forward pass, out[T] = cell_min * 2^(y + margin). With a list (list_n
+// entries, on the device) row i is tile list[i] instead of tile i.
 __global__ __launch_bounds__(256) void k_mlp_predict(Net N, TileSrc S,
                                                      float margin,
                                                      float* __restrict__ out,
-                                                     unsigned short* __restrict__ xdump) {
+                                                     unsigned short* __restrict__ xdump,
+                                                     const int* __restrict__ list,
+                                                     const int* __restrict__ list_n) {
   __shared__ __align__(16) unsigned short sX[kMlpRows * kXS];
   __shared__ __align__(16) unsigned short sA1[kMlpRows * kHS];
   __shared__ float sCM[kMlpRows];
   const long long ntiles = S.tile_off[S.ncells];
+  const long long nrows = list != nullptr ? *list_n : ntiles;
   const long long t0 = static_cast<long long>(blockIdx.x) * kMlpRows;
-  if (t0 >= ntiles) return;
+  if (t0 >= nrows) return;
+  // the tile of row i (ntiles, which no tile is, past the last row)
+  auto tile_of = [&](long long i) -> long long {
+    if (i >= nrows) return ntiles;
+    return list != nullptr ? static_cast<long long>(list[i]) : i;
+  };
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int lr = lane & 15, lg = lane >> 4;
   {
     // four threads per tile, a quarter of the inputs each
     const int rr = threadIdx.x >> 2, part = threadIdx.x & 3;
-    const TileCtx c = tile_ctx(S, t0 + rr, ntiles);
+    const TileCtx c = tile_ctx(S, tile_of(t0 + rr), ntiles);
     if (part == 0) sCM[rr] = static_cast<float>(c.cell_min);
     // occupancy inputs by (dz, dy) rows of five: thread `part` takes rows
     // part, part + 4, ...; the five dx offsets wrap once per thread, and
@@ -628,7 +637,7 @@ __global__ __launch_bounds__(256) void k_mlp_predict(Net N, TileSrc S,
   if (xdump != nullptr) {
     // diagnostics: the inputs exactly as the forward pass reads them
     for (int e = threadIdx.x; e < kMlpRows * kFeat; e += blockDim.x) {
-      const long long T = t0 + e / kFeat;
+      const long long T = tile_of(t0 + e / kFeat);
       if (T < ntiles) xdump[T * kFeat + e % kFeat] = sX[(e / kFeat) * kXS + e % kFeat];
     }
   }
@@ -639,7 +648,7 @@ __global__ __launch_bounds__(256) void k_mlp_predict(Net N, TileSrc S,
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = 16 * wv + 4 * lg + r;
-      const long long T = t0 + row;
+      const long long T = tile_of(t0 + row);
       if (T < ntiles) out[T] = sCM[row] * exp2f(y[r] + margin);
     }
   }
@@ -957,14 +966,15 @@ double Predictor::train(int epochs, int batch, double lr, unsigned seed,
 }
 
 void Predictor::predict_caps(const TileSrc& src, long long tcap, float* out,
-                             hipStream_t st, unsigned short* xdump) {
+                             hipStream_t st, unsigned short* xdump, const int* list,
+                             const int* list_n) {
   if (!trained_) throw std::runtime_error("the predictor is not trained");
   Net N{static_cast<const unsigned short*>(w1t_.p),
         static_cast<const unsigned short*>(w2t_.p),
         static_cast<const unsigned short*>(w2_.p),
         static_cast<const float*>(master_.p)};
   k_mlp_predict<<<blocks_for(tcap, kMlpRows), 256, 0, st>>>(
-      N, src, static_cast<float>(margin_), out, xdump);
+      N, src, static_cast<float>(margin_), out, xdump, list, list_n);
   check(hipGetLastError(), "mlp predict launch");
 }
 

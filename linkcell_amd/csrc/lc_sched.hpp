@@ -55,9 +55,11 @@ class Predictor {
   // 2^(y_T + margin) for T below the search's tile count (read on the
   // device), tcap the caller's upper bound of that count.
   // xdump (optional, device): the BF16 inputs of every tile as the
-  // forward pass reads them (diagnostics).
+  // forward pass reads them (diagnostics). list / list_n (optional,
+  // device): only the tiles list[0 .. *list_n) (at most tcap of them).
   void predict_caps(const TileSrc& src, long long tcap, float* out,
-                    hipStream_t st, unsigned short* xdump = nullptr);
+                    hipStream_t st, unsigned short* xdump = nullptr,
+                    const int* list = nullptr, const int* list_n = nullptr);
 
   // Append the tiles of a finished search with a finite label to the
   // training set (labels: device floats per tile, from the engine's
