@@ -1064,6 +1064,15 @@ __device__ __forceinline__ double raw_max(double a, double b) {
 #endif
 }
 
+// row_max16 of thresholds (never NaN: -inf, a key or a cap) through the
+// raw v_max_f64, without the quieting the compiler puts before fmax
+__device__ __forceinline__ double row_max16_thr(double v) {
+  v = raw_max(v, row_dpp<0xB1>(v));
+  v = raw_max(v, row_dpp<0x4E>(v));
+  v = raw_max(v, row_dpp<0x141>(v));
+  return raw_max(v, row_dpp<0x140>(v));
+}
+
 template <int KC, typename Reloc>
 __device__ __forceinline__ void merge8_keys(long long (&hd)[KC], long long (&b)[8],
                                             double& min_rej, Reloc&& reloc) {
@@ -2051,6 +2060,19 @@ __device__ __forceinline__ int row_wave0(const int* tile_off,
 #ifndef LC_KEYS_REG
 #define LC_KEYS_REG 1
 #endif
+#ifndef LC_RAW_THR
+#define LC_RAW_THR 1
+#endif
+#if LC_RAW_THR && LC_TAG_ASM
+#define LC_ROW_MAX_THR row_max16_thr
+#define LC_THR_MIN raw_min
+#else
+#define LC_ROW_MAX_THR row_max16
+#define LC_THR_MIN fmin
+#endif
+#ifndef LC_APPEND_ALL
+#define LC_APPEND_ALL 1
+#endif
 // Resident 256-thread blocks per CU the register budget is cut for: 3 (3
 // waves per SIMD, 168 VGPRs) for the heaps of up to 16, 2 (256 VGPRs) for
 // the 32-slot heap. Measured on the clustered 4M (k = 16), the uneven batch
@@ -2285,7 +2307,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     bool live = valid;
     double thr = live ? thr_cap : -inf;  // -inf: finished (never accepts)
     bool cap_fail = false;
-    double rthr = row_max16(live ? thr : -inf);  // -inf: no live source
+    double rthr = LC_ROW_MAX_THR(live ? thr : -inf);  // -inf: no live source
     int nbuf = 0;
     int done_reach = 0;
     unsigned st_tests = 0, st_tiles = 0, st_merges = 0, st_rounds = 0,
@@ -2354,9 +2376,9 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
         // from above (the accept test then takes a few more candidates of
         // the k-th's own tag bucket, never fewer)
         const long long wk = kTag ? (h.d[KC - 1] | TM) : h.d[KC - 1];
-        thr = live ? fmin(as_f64(wk), thr_cap) : -inf;
+        thr = live ? LC_THR_MIN(as_f64(wk), thr_cap) : -inf;
       }
-      rthr = row_max16(live ? thr : -inf);
+      rthr = LC_ROW_MAX_THR(live ? thr : -inf);
       if (kStats) cyc_merge += __builtin_readcyclecounter() - t0;
     };
 
@@ -2394,7 +2416,23 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
         qj = row_bcast<ii>(cj);
         if (SELF) acc = acc && qj != me;
       }
-      if (acc) {
+      if constexpr (kTag && LC_APPEND_ALL) {
+        // every lane writes its next free slot, and only an accepting lane
+        // claims it: a sweep pair starts with at most BC - 2 entries in
+        // every buffer, so the slot written is at most BC - 1 and stays
+        // in the buffer
+        const long long db = __double_as_longlong(d2);
+        s_rb.dw[wv][KC + 8 + nbuf][lane] = static_cast<unsigned>(db);
+        s_rb.dw[wv][KC + 16 + nbuf][lane] = static_cast<unsigned>(db >> 32);
+        s_rb.dw[wv][KC + nbuf][lane] = static_cast<unsigned>(qj);
+        nbuf += acc ? 1 : 0;
+        if (kStats && acc) {
+          st_acc += 1;
+          tile_acc = true;
+          if (ii < 8) tile_acc_lo = true;
+          else tile_acc_hi = true;
+        }
+      } else if (acc) {
         // indexed on the shared arrays themselves, so the stores stay LDS
         // stores whatever the register allocator does with pointers
         const long long db = __double_as_longlong(d2);
@@ -3037,7 +3075,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       // beyond r_safe the stencil wraps onto itself: exact walk
       if (live && r + 1 > d.r_safe) live = false;
       if (!live) thr = -inf;
-      rthr = row_max16(live ? thr : -inf);
+      rthr = LC_ROW_MAX_THR(live ? thr : -inf);
       if (!__ballot(live)) break;
     }
     const bool row_fail = ((__ballot(cap_fail) >> rbase) & 0xffffull) != 0;
@@ -3054,7 +3092,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       done_reach = 0;
       cap_fail = false;
     }
-    rthr = row_max16(live ? thr : -inf);
+    rthr = LC_ROW_MAX_THR(live ? thr : -inf);
     }
     if (kStats) {
       stat_add(A.stats, kStatRowTileTests, st_tests);
