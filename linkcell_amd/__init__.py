@@ -306,7 +306,8 @@ class Workspace:
                   "row_useful", "row_cyc_fill", "row_cyc_drain", "row_cyc_merge",
                   "row_cyc_all", "row_flushes", "row_waves_r2", "row_rounds_r2",
                   "row_useful_r2", "row_cyc_r2", "row_dead", "row_accepts",
-                  "row_dead_halves", "row_cyc_open", "row_cyc_init")
+                  "row_dead_halves", "row_cyc_open", "row_cyc_init", "row_idle_done",
+                  "row_idle_empty")
 
     def last_stats(self) -> dict:
         n = len(self.STAT_NAMES)

@@ -1379,7 +1379,9 @@ enum {
   kStatRowDeadHalf, // row-level 8-candidate halves of swept tiles in which no source accepted
   kStatRowCycOpen,  // wave-level cycles of the opening sweeps (own and next tile)
   kStatRowCycInit,  // wave-level cycles before the opening
-  kStatCount = 33
+  kStatRowIdleDone, // row-level drain rounds idle because the row has no live source
+  kStatRowIdleEmpty,  // row-level drain rounds idle because the live row listed nothing
+  kStatCount = 35
 };
 
 __device__ __forceinline__ void stat_add(unsigned long long* st, int i,
@@ -2316,6 +2318,7 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
     unsigned st_flush = 0;
     unsigned st_rounds2 = 0, st_useful2 = 0, st_wave2 = 0;
     unsigned st_dead = 0, st_acc = 0, st_deadh = 0;
+    unsigned st_idle_done = 0, st_idle_empty = 0;
     bool tile_acc = false, tile_acc_lo = false, tile_acc_hi = false;
     unsigned long long cyc_r2 = 0, cyc_open = 0, cyc_init = 0;
     int cur_r = 1;
@@ -2678,6 +2681,12 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
         cz = nzq + shz;
         cj = njq;
         if (kStats) st_useful += e >= 0 ? 1 : 0;
+        if (kStats && e < 0) {
+          // an idle row: finished, or live with an empty list this drain
+          // (the rest ran out of listed tiles before the wave's last round)
+          if (!(rthr >= 0.0)) st_idle_done += 1;
+          else if (lcount == 0) st_idle_empty += 1;
+        }
         if (kStats && cur_r >= 2) {
           st_rounds2 += 1;
           st_useful2 += e >= 0 ? 1 : 0;
@@ -3120,6 +3129,8 @@ __global__ __launch_bounds__(256, LC_ROW_BLOCKS(KC)) void k_walk_rows(WalkArgs A
       if (li == 0) stat_add(A.stats, kStatRowUsefulR2, st_useful2);
       if (li == 0) stat_add(A.stats, kStatRowDead, st_dead);
       if (li == 0) stat_add(A.stats, kStatRowDeadHalf, st_deadh);
+      if (li == 0) stat_add(A.stats, kStatRowIdleDone, st_idle_done);
+      if (li == 0) stat_add(A.stats, kStatRowIdleEmpty, st_idle_empty);
       stat_add(A.stats, kStatRowAccepts, st_acc);
     }
     if (!valid) continue;
